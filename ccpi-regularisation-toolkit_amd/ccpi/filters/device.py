@@ -1,0 +1,147 @@
+"""Device-resident solves on torch tensors (no host copies).
+
+The reference offers this only through CuPy for ROF/PD
+(src/Python/ccpi/filters/regularisersCuPy.py:35-256).  Here every method takes
+a float32 (or float64 for the ``*_f64`` solvers) CUDA tensor in (Z, Y, X) or
+(Y, X) order and returns ``(out, infovector)``; the work is queued on the
+tensor device's current torch stream and the call returns when it is done.
+
+Options beyond the reference (see include/cilreg_ext.h):
+``check_every`` (residual-check cadence, reference 5), ``precision``
+('fp32', 'bf16', 'fp16' storage of the dual/auxiliary fields), ``tile_y`` and
+``zchunk`` (tiling; 0 = automatic).
+"""
+import ctypes
+
+import numpy as np
+
+from .utils import require_gpu_lib, last_error
+
+PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
+
+
+class CilregOpts(ctypes.Structure):
+    _fields_ = [("check_every", ctypes.c_int), ("dual_precision", ctypes.c_int), ("tile_y", ctypes.c_int),
+                ("zchunk", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+
+
+def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0):
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk))
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_d = ctypes.c_double
+_SIGS = {
+    "cilreg_rof_dev": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_fgp_dev": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _i, _vp, _vp],
+    "cilreg_pd_dev": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
+    "cilreg_sb_dev": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _vp, _vp],
+    "cilreg_tgv_dev": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_fgp_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _i, _i, _i, _i, _i, _vp, _vp],
+    "cilreg_tgv_dev_f64": [_vp, _vp, _vp, _d, _d, _d, _i, _d, _d, _i, _i, _i, _vp, _vp],
+}
+
+
+def _fn(name):
+    lib = require_gpu_lib()
+    f = getattr(lib, name)
+    f.argtypes = _SIGS[name]
+    f.restype = _i
+    return f
+
+
+def _dims(t):
+    d = list(t.shape)[::-1]
+    if t.dim() == 2:
+        d.append(1)
+    if len(d) != 3:
+        raise ValueError(f"only 2D and 3D tensors are supported, got shape {tuple(t.shape)}")
+    return d
+
+
+def _prep(x, out, dtype):
+    import torch
+    if not (x.is_cuda and x.dtype == dtype and x.is_contiguous()):
+        raise ValueError(f"input must be a contiguous {dtype} CUDA tensor")
+    if out is None:
+        out = torch.zeros_like(x)
+    elif not (out.is_cuda and out.dtype == dtype and out.is_contiguous() and out.shape == x.shape
+              and out.device == x.device):
+        raise ValueError("out must be a contiguous tensor of the input's shape, dtype and device")
+    return out
+
+
+def _run(name, x, out, dtype, scalars, opts):
+    import torch
+    info = np.zeros(2, dtype=np.float64 if dtype == torch.float64 else np.float32)
+    f = _fn(name)
+    with torch.cuda.device(x.device):
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        rc = f(x.data_ptr(), out.data_ptr(), info.ctypes.data, *scalars, *_dims(x), ctypes.byref(opts),
+               ctypes.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError(f"{name} failed: {last_error()}")
+    return out, info
+
+
+def rof(x, lam, iters, tau, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_rof_dev", x, out, torch.float32, [lam, iters, tau, tol], make_opts(**kw))
+
+
+def fgp(x, lam, iters, tol=0.0, method=0, nonneg=0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_fgp_dev", x, out, torch.float32, [lam, iters, tol, method, nonneg], make_opts(**kw))
+
+
+def pd(x, lam, iters, tol=0.0, L=12.0, method=0, nonneg=0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_pd_dev", x, out, torch.float32, [lam, iters, tol, L, method, nonneg], make_opts(**kw))
+
+
+def sb(x, mu, iters, tol=0.0, method=0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_sb_dev", x, out, torch.float32, [mu, iters, tol, method], make_opts(**kw))
+
+
+def tgv(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_tgv_dev", x, out, torch.float32, [lam, alpha1, alpha0, iters, L2, tol], make_opts(**kw))
+
+
+def fgp_f64(x, lam, iters, tol=0.0, method=0, nonneg=0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float64)
+    return _run("cilreg_fgp_dev_f64", x, out, torch.float64, [lam, iters, tol, method, nonneg], make_opts(**kw))
+
+
+def tgv_f64(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float64)
+    return _run("cilreg_tgv_dev_f64", x, out, torch.float64, [lam, alpha1, alpha0, iters, L2, tol],
+                make_opts(**kw))
+
+
+def run(method, x, params, out=None, **kw):
+    """Dispatch by method name with a parameter dict (keys of the oracle's run())."""
+    p = params
+    if method == "rof":
+        return rof(x, p["lambda"], p["iters"], p["tau"], p["tol"], out=out, **kw)
+    if method == "fgp":
+        return fgp(x, p["lambda"], p["iters"], p["tol"], p["method"], p["nonneg"], out=out, **kw)
+    if method == "pd":
+        return pd(x, p["lambda"], p["iters"], p["tol"], p["L"], p["method"], p["nonneg"], out=out, **kw)
+    if method == "sb":
+        return sb(x, p["mu"], p["iters"], p["tol"], p["method"], out=out, **kw)
+    if method == "tgv":
+        return tgv(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], out=out, **kw)
+    raise ValueError(method)

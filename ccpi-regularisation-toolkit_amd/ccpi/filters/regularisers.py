@@ -1,0 +1,43 @@
+"""User API: ``ROF_TV, FGP_TV, PD_TV, SB_TV, TGV`` with ``device=`` dispatch.
+
+Mirrors src/Python/ccpi/filters/regularisers.py:19-49 of the reference:
+``device='cpu'`` calls the C-OMP function, ``'gpu'`` or an integer device
+index calls the MI355X function with ``gpu_device`` set, anything else raises
+KeyError.  Positional argument order per method:
+
+* ``ROF_TV(x, lambda, iterations, tau, tol)``
+* ``FGP_TV(x, lambda, iterations, tol, methodTV, nonneg)``
+* ``PD_TV(x, lambda, iterations, tol, lipschitz_const, methodTV, nonneg)``
+* ``SB_TV(x, mu, iterations, tol, methodTV)``
+* ``TGV(x, lambda, alpha1, alpha0, iterations, L2, tol)``
+
+each also taking ``out=`` and ``infovector=`` (``Output=`` for PD on the GPU,
+as in the reference, TV.py:567).
+"""
+import numbers
+from functools import wraps
+
+from .TV import TV_ROF_CPU, TV_ROF_GPU
+from .TV import TV_FGP_CPU, TV_FGP_GPU
+from .TV import PDTV_CPU, PDTV_GPU
+from .TV import SB_TV_CPU, SB_TV_GPU
+from .TV import TGV_CPU, TGV_GPU
+from .utils import cilregcuda
+
+
+def create_wrapper(CPU_func, GPU_func):
+    @wraps(CPU_func)
+    def wrapper(*args, device='cpu', **kwargs):
+        if device == 'cpu':
+            return CPU_func(*args, **kwargs)
+        elif device == 'gpu' or isinstance(device, numbers.Integral) and cilregcuda is not None:
+            return GPU_func(*args, gpu_device=0 if device == 'gpu' else device, **kwargs)
+        raise KeyError(f"{GPU_func.__name__}: device {device} not available")
+    return wrapper
+
+
+ROF_TV = create_wrapper(TV_ROF_CPU, TV_ROF_GPU)
+FGP_TV = create_wrapper(TV_FGP_CPU, TV_FGP_GPU)
+PD_TV = create_wrapper(PDTV_CPU, PDTV_GPU)
+SB_TV = create_wrapper(SB_TV_CPU, SB_TV_GPU)
+TGV = create_wrapper(TGV_CPU, TGV_GPU)

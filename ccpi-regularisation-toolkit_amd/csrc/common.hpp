@@ -1,0 +1,190 @@
+// common.hpp -- shared device/host machinery of the MI355X regulariser library.
+//
+// Layout of every field in HBM: one dense array per channel, index
+// (nx*ny)*z + nx*y + x with x fastest (the reference's layout,
+// src/Core/regularisers_CPU/FGP_TV_core.c:228).  Kernels march a 64 x TY
+// (x, y) tile of threads through a chunk of z planes ("2.5D" marching): each
+// thread owns one (x, y) column, values a plane needs from its z-predecessor
+// stay in registers, and the per-plane intermediate results that neighbouring
+// threads need (the updated dual field, the FGP objective, the ROF fluxes) go
+// through LDS.  Inputs are read straight from HBM with neighbour re-reads
+// served by L1/L2; blocks are remapped so each XCD walks a contiguous run of
+// tiles (its L2 then holds the halos of its own tiles).
+//
+// Arithmetic is kept operation-for-operation identical to the reference C-OMP
+// code (the library is compiled with -ffp-contract=off, IEEE division and
+// sqrt), including its double promotions, so FP32 results match the CPU
+// oracle bit for bit; only the stopping-rule norms differ (FP64 tree sums here,
+// a serial FP32 sum in the reference).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#pragma clang fp contract(off)
+
+namespace cilreg {
+
+constexpr int TX = 64;          // tile width in x = one wavefront
+constexpr int NXCD = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// ---------------------------------------------------------------------------
+// storage types for dual / auxiliary fields
+// ---------------------------------------------------------------------------
+struct bf16_t { uint16_t bits; };
+
+template <typename S> struct Store;
+template <> struct Store<float> {
+    __device__ static inline float ld(const float* p, size_t i) { return p[i]; }
+    __device__ static inline void st(float* p, size_t i, float v) { p[i] = v; }
+};
+template <> struct Store<double> {
+    __device__ static inline double ld(const double* p, size_t i) { return p[i]; }
+    __device__ static inline void st(double* p, size_t i, double v) { p[i] = v; }
+};
+template <> struct Store<_Float16> {
+    __device__ static inline float ld(const _Float16* p, size_t i) { return (float)p[i]; }
+    __device__ static inline void st(_Float16* p, size_t i, float v) { p[i] = (_Float16)v; }  // RNE
+};
+template <> struct Store<bf16_t> {
+    __device__ static inline float ld(const bf16_t* p, size_t i) {
+        return __uint_as_float(((uint32_t)p[i].bits) << 16);
+    }
+    __device__ static inline void st(bf16_t* p, size_t i, float v) {
+        uint32_t u = __float_as_uint(v);
+        uint16_t r;
+        if ((u & 0x7fffffffu) > 0x7f800000u) r = (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
+        else r = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);               // round to nearest even
+        p[i].bits = r;
+    }
+};
+
+template <typename T, typename S>
+__device__ inline T ldT(const S* p, size_t i) { return (T)Store<S>::ld(p, i); }
+template <typename T, typename S>
+__device__ inline void stT(S* p, size_t i, T v) { Store<S>::st(p, i, v); }
+
+// ---------------------------------------------------------------------------
+// volume geometry and the block -> tile mapping
+// ---------------------------------------------------------------------------
+struct Geom {
+    int nx, ny, nz;       // volume extent (nz = 1 for 2D)
+    int tiles_x, tiles_y; // xy tiles of TX x ty
+    int zc;               // planes per z chunk
+    int nzc;              // number of z chunks
+    int nblocks;          // tiles_x * tiles_y * nzc
+    long long sxy;        // nx * ny
+};
+
+inline Geom make_geom(int nx, int ny, int nz, int ty, int zc) {
+    Geom g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.tiles_x = (nx + TX - 1) / TX;
+    g.tiles_y = (ny + ty - 1) / ty;
+    if (zc < 1) zc = 1;
+    if (zc > nz) zc = nz;
+    g.zc = zc;
+    g.nzc = (nz + zc - 1) / zc;
+    g.nblocks = g.tiles_x * g.tiles_y * g.nzc;
+    g.sxy = (long long)nx * ny;
+    return g;
+}
+
+// Hardware deals workgroup b to XCD b % 8.  Remap so XCD x receives the
+// contiguous logical range [x*q + min(x, r), ...): its tiles are neighbours and
+// their halos hit its own L2.
+__device__ inline int xcd_logical_block(int b, int nb) {
+    int q = nb / NXCD, r = nb % NXCD;
+    int x = b % NXCD, s = b / NXCD;
+    return x * q + (x < r ? x : r) + s;
+}
+
+struct TileCoord { int x0, y0, z0, z1; int lin; };
+
+__device__ inline TileCoord tile_of_block(const Geom& g, int ty) {
+    TileCoord t;
+    int L = xcd_logical_block((int)blockIdx.x, g.nblocks);
+    t.lin = L;
+    int txi = L % g.tiles_x;
+    int rem = L / g.tiles_x;
+    int tyi = rem % g.tiles_y;
+    int zci = rem / g.tiles_y;
+    t.x0 = txi * TX;
+    t.y0 = tyi * ty;
+    t.z0 = zci * g.zc;
+    t.z1 = t.z0 + g.zc < g.nz ? t.z0 + g.zc : g.nz;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// stopping rule: per-block FP64 partial sums, a fixed-order final reduction
+// ---------------------------------------------------------------------------
+// Device control block of one solve.  The reference's rule
+// (ROF_TV_core.c:79-89): every check, re = |u - u_prev| / |u|; count the
+// checks with re < epsil (cumulative) and stop when count > 3.
+struct Ctrl {
+    int stopped;     // 1 once the rule fired; every later kernel returns at once
+    int stop_iter;   // the iteration index at which it fired
+    int count;       // cumulative number of sub-tolerance checks
+    float re;        // last residual, as the reference reports it
+    double sum_d, sum_u;  // last check's sums (diagnostics)
+};
+
+// Sum (a, b) over a 256-thread block in a fixed order; result valid in thread 0.
+__device__ inline void block_sum2(double& a, double& b, double* lds /* >= 8 doubles */) {
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int w = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
+    if ((tid & 63) == 0) { lds[2 * w] = a; lds[2 * w + 1] = b; }
+    __syncthreads();
+    if (tid == 0) {
+        double sa = 0, sb = 0;
+        for (int i = 0; i < nw; ++i) { sa += lds[2 * i]; sb += lds[2 * i + 1]; }
+        a = sa; b = sb;
+    }
+}
+
+// The flag is written by an earlier kernel of the same stream, so a plain
+// load at kernel start sees it.
+__device__ inline bool solve_stopped(const Ctrl* c) { return c != nullptr && c->stopped != 0; }
+
+// ---------------------------------------------------------------------------
+// exact restatements of the reference's mixed-precision expressions
+// ---------------------------------------------------------------------------
+template <typename T> struct Mixed;
+template <> struct Mixed<float> {
+    // sqrtf((float)((double)s + 1e-8)): ROF_TV_core.c:160,199,223,262
+    __device__ static inline float rof_sqrt_f(float s) { return sqrtf((float)((double)s + 1.0e-8)); }
+    // (float)sqrt((double)s + 1e-8): the 3D D1, ROF_TV_core.c:136
+    __device__ static inline float rof_sqrt_d(float s) { return (float)sqrt((double)s + 1.0e-8); }
+    // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
+    __device__ static inline float pd_div(float num, float lt) { return (float)((double)num / (1.0 + (double)lt)); }
+    // squared minmod, ROF_TV_core.c:132-133 (fabs promotes to double, exact here)
+    __device__ static inline float minmod_sq(float a, float b) {
+        int sa = (a > 0.f) - (a < 0.f), sb = (b > 0.f) - (b < 0.f);
+        double fa = fabs((double)a), fb = fabs((double)b);
+        float m = (float)((0.5f * (float)(sa + sb)) * (fa < fb ? fa : fb));
+        return m * m;
+    }
+};
+template <> struct Mixed<double> {
+    __device__ static inline double rof_sqrt_f(double s) { return sqrt(s + 1.0e-8); }
+    __device__ static inline double rof_sqrt_d(double s) { return sqrt(s + 1.0e-8); }
+    __device__ static inline double pd_div(double num, double lt) { return num / (1.0 + lt); }
+    __device__ static inline double minmod_sq(double a, double b) {
+        int sa = (a > 0.0) - (a < 0.0), sb = (b > 0.0) - (b < 0.0);
+        double fa = fabs(a), fb = fabs(b);
+        double m = (0.5 * (double)(sa + sb)) * (fa < fb ? fa : fb);
+        return m * m;
+    }
+};
+
+template <typename T> __device__ inline T dsqrt(T x);
+template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
+
+}  // namespace cilreg

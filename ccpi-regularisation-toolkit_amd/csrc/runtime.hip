@@ -1,0 +1,204 @@
+// runtime.hip -- host runtime and the small service kernels of the library.
+#include "runtime.hpp"
+#include <stdarg.h>
+#include <string.h>
+#include <mutex>
+
+namespace cilreg {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "cilregcuda: %s\n", g_err);
+}
+
+const char* last_error() { return g_err; }
+
+int select_device(int dev) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP devices found (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+        return -1;
+    }
+    if (dev < 0 || dev >= n) {
+        set_error("invalid gpu_device %d (%d device(s) present)", dev, n);
+        return -1;
+    }
+    CILREG_CK(hipSetDevice(dev));
+    return 0;
+}
+
+hipStream_t device_stream() {
+    static std::mutex mu;
+    static hipStream_t streams[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!streams[dev]) {
+        if (hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess) streams[dev] = nullptr;
+    }
+    return streams[dev];
+}
+
+Arena::~Arena() {
+    if (base_) (void)hipFree(base_);
+}
+
+size_t Arena::reserve(size_t bytes) {
+    size_t off = total_;
+    total_ += (bytes + 255) & ~(size_t)255;
+    return off;
+}
+
+int Arena::allocate() {
+    if (total_ == 0) total_ = 256;
+    hipError_t e = hipMalloc(&base_, total_);
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        set_error("hipMalloc of %zu bytes failed: %s", total_, hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+PinnedCtrl::~PinnedCtrl() {
+    if (h) (void)hipHostFree(h);
+}
+
+int PinnedCtrl::allocate() {
+    CILREG_CK(hipHostMalloc((void**)&h, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    memset(h, 0, 2 * sizeof(Ctrl));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// service kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void finalize_kernel(const double* partials, int nblocks, Ctrl* ctrl, float epsil,
+                                                       int iter) {
+    __shared__ double red[2 * 256];
+    if (ctrl->stopped) return;
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    red[threadIdx.x] = a;
+    red[256 + threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[threadIdx.x] += red[threadIdx.x + s];
+            red[256 + threadIdx.x] += red[256 + threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sd = red[0], su = red[256];
+        // re = sqrt(sum d^2) / sqrt(sum u^2), as ROF_TV_core.c:86 (0/0 -> NaN never stops)
+        float re = (float)(sqrt(sd) / sqrt(su));
+        ctrl->re = re;
+        ctrl->sum_d = sd;
+        ctrl->sum_u = su;
+        int count = ctrl->count;
+        if (re < epsil) count++;
+        ctrl->count = count;
+        if (count > 3) {
+            ctrl->stop_iter = iter;
+            ctrl->stopped = 1;
+        }
+    }
+}
+
+__global__ void ctrl_reset_kernel(Ctrl* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl->stopped = 0;
+        ctrl->stop_iter = 0;
+        ctrl->count = 0;
+        ctrl->re = 0.f;
+        ctrl->sum_d = 0;
+        ctrl->sum_u = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(uint32_t* p, size_t nwords) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(uint32_t* d, const uint32_t* s, size_t nwords) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < nwords; i += stride) d[i] = s[i];
+}
+
+__global__ void tail_bytes_kernel(unsigned char* d, const unsigned char* s, size_t n) {
+    // n < 4: zero (s == null) or copy the last bytes
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) d[i] = s ? s[i] : 0;
+}
+
+void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, ctrl, epsil, iter);
+}
+
+void launch_ctrl_reset(Ctrl* ctrl, hipStream_t s) {
+    hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, ctrl);
+}
+
+static unsigned grid_for(size_t nwords) {
+    size_t b = (nwords + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+    size_t nw = bytes / 4;
+    if (nw) hipLaunchKernelGGL(zero_kernel, dim3(grid_for(nw)), dim3(256), 0, s, (uint32_t*)p, nw);
+    if (bytes & 3)
+        hipLaunchKernelGGL(tail_bytes_kernel, dim3(1), dim3(64), 0, s, (unsigned char*)p + nw * 4,
+                           (const unsigned char*)nullptr, bytes & 3);
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    size_t nw = bytes / 4;
+    if (nw) hipLaunchKernelGGL(copy_kernel, dim3(grid_for(nw)), dim3(256), 0, s, (uint32_t*)dst, (const uint32_t*)src, nw);
+    if (bytes & 3)
+        hipLaunchKernelGGL(tail_bytes_kernel, dim3(1), dim3(64), 0, s, (unsigned char*)dst + nw * 4,
+                           (const unsigned char*)src + nw * 4, bytes & 3);
+}
+
+Opts resolve_opts(const cilreg_opts* o) {
+    Opts r;
+    if (o) {
+        if (o->check_every > 0) r.check_every = o->check_every;
+        r.precision = o->dual_precision;
+        r.tile_y = o->tile_y;
+        r.zchunk = o->zchunk;
+    }
+    return r;
+}
+
+void auto_tiling(int nx, int ny, int nz, int* ty, int* zc) {
+    int t = (*ty == 4 || *ty == 8) ? *ty : 4;
+    *ty = t;
+    if (*zc > 0) return;
+    if (nz <= 1) { *zc = 1; return; }
+    long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + t - 1) / t);
+    // aim for >= 2048 workgroups (8 per CU) while marching as deep as possible
+    long long want_chunks = (2048 + tiles - 1) / tiles;
+    if (want_chunks < 1) want_chunks = 1;
+    int c = (int)((nz + want_chunks - 1) / want_chunks);
+    if (c < 4) c = 4;
+    if (c > 64) c = 64;
+    if (c > nz) c = nz;
+    *zc = c;
+}
+
+}  // namespace cilreg

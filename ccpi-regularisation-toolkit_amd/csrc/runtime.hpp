@@ -1,0 +1,145 @@
+// runtime.hpp -- host-side runtime of the regulariser library: device
+// selection, error reporting, device buffers, the stopping-rule finaliser and
+// the iteration driver shared by every solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdint.h>
+#include <vector>
+#include <math.h>
+#include "common.hpp"
+#include "../../include/cilreg_ext.h"
+
+namespace cilreg {
+
+// Records the message as the thread's last error and prints it to stderr, as
+// the reference's CHECK / checkCudaErrors macros do (regularisers_GPU/shared.h:15-38).
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+const char* last_error();
+
+#define CILREG_CK(call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::cilreg::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,          \
+                                hipGetErrorString(e_));                                   \
+            return -1;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+// hipSetDevice(dev) after checking the device count; -1 with a message otherwise.
+int select_device(int dev);
+
+// The library's own stream on the current device (default priority, created once).
+hipStream_t device_stream();
+
+// One device allocation carved into aligned pieces; freed on destruction, so
+// every exit path of a solve releases its memory.
+class Arena {
+public:
+    Arena() = default;
+    ~Arena();
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    // Reserve bytes (256-byte aligned) before allocate(); returns the offset.
+    size_t reserve(size_t bytes);
+    int allocate();                       // one hipMalloc for everything reserved
+    void* at(size_t off) const { return (char*)base_ + off; }
+    size_t bytes() const { return total_; }
+private:
+    void* base_ = nullptr;
+    size_t total_ = 0;
+};
+
+// Pinned host mirror of the control block (for stop polling).
+class PinnedCtrl {
+public:
+    ~PinnedCtrl();
+    int allocate();
+    Ctrl* h = nullptr;
+};
+
+// Launch the fixed-order reduction of per-block partials that applies the
+// stopping rule for iteration `iter` (one block; deterministic).
+void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s);
+// Zero the control block (a small fill kernel, safe under graph capture).
+void launch_ctrl_reset(Ctrl* ctrl, hipStream_t s);
+// Fill a float/double/16-bit array with zeros, or copy one array to another.
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+inline float dsqrt_host(float x) { return sqrtf(x); }
+inline double dsqrt_host(double x) { return sqrt(x); }
+
+// Per-solve options resolved from the public cilreg_opts.
+struct Opts {
+    int check_every = 5;   // the reference checks every 5th iteration
+    int precision = CILREG_FP32;
+    int tile_y = 0;        // 0: auto
+    int zchunk = 0;        // 0: auto
+    int poll = 16;         // host polls the stop flag every `poll` iterations
+};
+Opts resolve_opts(const cilreg_opts* o);
+
+// Default tile height and z-chunk for a volume so the grid has enough blocks
+// to cover 256 CUs several times.
+void auto_tiling(int nx, int ny, int nz, int* ty, int* zc);
+
+// Drives `iter` iterations on stream s.  step(k, check) launches iteration k
+// (check: whether this iteration evaluates the stopping rule); the driver then
+// launches the finaliser, polls the stop flag every opts.poll iterations
+// without draining the queue, and on return has synchronised the stream and
+// filled *iters_done and *re.
+template <class StepFn>
+int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, double* d_partials,
+                     int nblocks, hipStream_t s, StepFn&& step, int* iters_done, float* re) {
+    PinnedCtrl pin;
+    const bool use_tol = epsil != 0.0f;
+    if (use_tol && pin.allocate() != 0) return -1;
+    launch_ctrl_reset(d_ctrl, s);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int flag_slot_pending[2] = {0, 0};
+    if (use_tol) {
+        CILREG_CK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        CILREG_CK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+    int poll_idx = 0;
+    int rc = 0;
+    for (int k = 0; k < iter; ++k) {
+        const bool check = use_tol && (k % opts.check_every == 0);
+        step(k, check);
+        if (check) launch_finalize(d_partials, nblocks, d_ctrl, epsil, k, s);
+        if (use_tol && ((k + 1) % opts.poll == 0) && k + 1 < iter) {
+            const int slot = poll_idx & 1;
+            // pinned slot: h[slot].stopped
+            if (hipMemcpyAsync(&pin.h[slot].stopped, &d_ctrl->stopped, sizeof(int), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipEventRecord(ev[slot], s) != hipSuccess) {
+                set_error("stop-flag poll failed");
+                rc = -1;
+                break;
+            }
+            flag_slot_pending[slot] = 1;
+            const int other = slot ^ 1;
+            if (flag_slot_pending[other]) {
+                if (hipEventSynchronize(ev[other]) != hipSuccess) { set_error("event sync failed"); rc = -1; break; }
+                flag_slot_pending[other] = 0;
+                if (pin.h[other].stopped) break;   // the remaining launches would be no-ops
+            }
+            ++poll_idx;
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (rc == 0 && e != hipSuccess) { set_error("kernel launch failed: %s", hipGetErrorString(e)); rc = -1; }
+    e = hipStreamSynchronize(s);
+    if (rc == 0 && e != hipSuccess) { set_error("solve failed: %s", hipGetErrorString(e)); rc = -1; }
+    for (int i = 0; i < 2; ++i) if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (rc != 0) return rc;
+    Ctrl c;
+    CILREG_CK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    *iters_done = c.stopped ? c.stop_iter : iter;
+    *re = c.re;
+    return 0;
+}
+
+}  // namespace cilreg

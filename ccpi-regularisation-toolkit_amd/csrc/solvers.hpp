@@ -1,0 +1,28 @@
+// solvers.hpp -- device-resident solver entry points shared by the C ABI, the
+// extension API, the batch launcher and the serving loop.
+//
+// Every solver takes device pointers and a stream, runs `iter` iterations of
+// its method with the reference's stopping rule, writes the reference's
+// output into d_out and reports the iteration count and last residual.
+#pragma once
+#include "runtime.hpp"
+
+namespace cilreg {
+
+template <typename T, typename S>
+int rof_solve(const T* d_f, T* d_out, T lambda, int iter, T tau, T epsil, int nx, int ny, int nz, const Opts& opts,
+              hipStream_t s, int* iters_done, float* re);
+template <typename T, typename S>
+int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, int nonneg, int nx, int ny, int nz,
+              const Opts& opts, hipStream_t s, int* iters_done, float* re);
+template <typename T, typename S>
+int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, int method, int nonneg, int nx, int ny,
+             int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);
+template <typename T, typename S>
+int sb_solve(const T* d_f, T* d_out, T mu, int iter, T epsil, int method, int nx, int ny, int nz, const Opts& opts,
+             hipStream_t s, int* iters_done, float* re);
+template <typename T, typename S>
+int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
+              const Opts& opts, hipStream_t s, int* iters_done, float* re);
+
+}  // namespace cilreg

@@ -1,0 +1,334 @@
+// tgv.hip -- TGV^2-L2 denoising by primal-dual iterations, one fused kernel
+// per iteration.
+//
+// Reference algorithm: src/Core/regularisers_CPU/TGV_core.c:43-568.  State:
+// the extrapolated primal u_bar, the extrapolated auxiliary field v_bar
+// (3 channels; 2 in 2D), the duals P (3; 2) and Q (6 symmetric-tensor
+// channels Q11,Q22,Q33,Q12,Q13,Q23; 3 in 2D).  tau = sigma = L2^-1/2.
+//   P = proj_alpha1(P + sigma (grad+ u_bar - v_bar))    (DualP/ProjP, :351-387)
+//   Q = proj_alpha0(Q + sigma sym-grad+ v_bar)           (DualQ/ProjQ, :389-446)
+//   U = (lambda (u_bar + tau div P) + tau f) / (lambda + tau)    (:448-472)
+//   u_bar <- 2 U - u_bar                                 (:548-554)
+//   V = v_bar + tau (P + div Q);  v_bar <- 2 V - v_bar   (:474-533, :558-568)
+// Forward differences are zero at the last index; the divergences are their
+// adjoints (the field at index 0, minus the predecessor at the last index).
+// The residual compares the new u_bar with the old one.
+//
+// HBM traffic per voxel-iteration (FP32): read u_bar, f, v_bar(3), P(3),
+// Q(6) = 14 fields, write 13 = 108 B.  With 16-bit storage of v_bar, P and Q:
+// 60 B.
+//
+// Fusion: for plane z the updated P and Q are computed on the tile plus its
+// -x / -y halo; the channels the backward divergences need from a neighbour
+// go through LDS, those needed from plane z-1 stay in registers.
+#include "solvers.hpp"
+
+namespace cilreg {
+
+template <typename T, typename S>
+struct TgvArgs {
+    const T* f;
+    const T* U;
+    const S* V[3];
+    const S* P[3];
+    const S* Q[6];
+    T* Uout;
+    S* Vout[3];
+    S* Pout[3];
+    S* Qout[6];
+    T lambda, tau, sigma, alpha1, alpha0;
+    int check;
+    double* partials;
+    const Ctrl* ctrl;
+    Geom g;
+};
+
+// LDS channels shared with the +x neighbour (backward difference along x)
+// and the +y neighbour (along y).
+enum { L_P1, L_P2, L_Q11, L_Q22, L_Q12, L_Q13, L_Q23, L_N };
+
+template <int DIM, typename T, typename S>
+__device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, T p[3], T q[6]) {
+    const Geom& g = a.g;
+    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
+    const size_t i = (size_t)z * sxy + (size_t)py * nx + px;
+    const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
+    const T sigma = a.sigma;
+    const T u = a.U[i];
+    T v[3], vx[3], vy[3], vz[3];
+    #pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        v[c] = ldT<T>(a.V[c], i);
+        vx[c] = lx ? v[c] : ldT<T>(a.V[c], i + 1);
+        vy[c] = ly ? v[c] : ldT<T>(a.V[c], i + nx);
+        if (DIM == 3) vz[c] = lz ? v[c] : ldT<T>(a.V[c], i + sxy);
+    }
+    // P (TGV_core.c:360-365): forward difference of u_bar minus v_bar
+    p[0] = ldT<T>(a.P[0], i) + sigma * (lx ? -v[0] : (a.U[i + 1] - u) - v[0]);
+    p[1] = ldT<T>(a.P[1], i) + sigma * (ly ? -v[1] : (a.U[i + nx] - u) - v[1]);
+    T nrm;
+    if (DIM == 3) {
+        p[2] = ldT<T>(a.P[2], i) + sigma * (lz ? -v[2] : (a.U[i + sxy] - u) - v[2]);
+        nrm = dsqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / a.alpha1;
+    } else {
+        p[2] = 0;
+        nrm = dsqrt(p[0] * p[0] + p[1] * p[1]) / a.alpha1;
+    }
+    if (nrm > (T)1) {
+        p[0] /= nrm;
+        p[1] /= nrm;
+        if (DIM == 3) p[2] /= nrm;
+    }
+    // Q (TGV_core.c:398-421): d[a][b] = forward difference of V_a along axis b
+    T d[3][3];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) d[c][0] = d[c][1] = d[c][2] = 0;
+    #pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        if (!lx) d[c][0] = vx[c] - v[c];
+        if (!ly) d[c][1] = vy[c] - v[c];
+        if (DIM == 3 && !lz) d[c][2] = vz[c] - v[c];
+    }
+    if (DIM == 3) {
+        q[0] = ldT<T>(a.Q[0], i) + sigma * d[0][0];
+        q[1] = ldT<T>(a.Q[1], i) + sigma * d[1][1];
+        q[2] = ldT<T>(a.Q[2], i) + sigma * d[2][2];
+        q[3] = ldT<T>(a.Q[3], i) + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
+        q[4] = ldT<T>(a.Q[4], i) + sigma * ((T)0.5 * (d[2][0] + d[0][2]));
+        q[5] = ldT<T>(a.Q[5], i) + sigma * ((T)0.5 * (d[2][1] + d[1][2]));
+        T m = dsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + (T)2 * q[3] * q[3] + (T)2 * q[4] * q[4] +
+                    (T)2 * q[5] * q[5]);
+        m = m / a.alpha0;
+        if (m > (T)1) {
+            #pragma unroll
+            for (int c = 0; c < 6; ++c) q[c] /= m;
+        }
+    } else {
+        q[0] = ldT<T>(a.Q[0], i) + sigma * d[0][0];
+        q[1] = ldT<T>(a.Q[1], i) + sigma * d[1][1];
+        q[2] = ldT<T>(a.Q[2], i) + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
+        q[3] = q[4] = q[5] = 0;
+        T m = dsqrt(q[0] * q[0] + q[1] * q[1] + (T)2 * q[2] * q[2]);
+        m = m / a.alpha0;
+        if (m > (T)1) {
+            q[0] /= m;
+            q[1] /= m;
+            q[2] /= m;
+        }
+    }
+}
+
+// adjoint difference: own value at index 0, minus the predecessor at the last
+// index, plain backward difference in between (TGV_core.c:458-466)
+template <typename T>
+__device__ inline T bdiff(T own, T pred, int pos, int len) {
+    if (pos == 0) return own;
+    if (pos == len - 1) return -pred;
+    return own - pred;
+}
+
+template <int DIM, int TY, typename T, typename S>
+__global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
+    if (solve_stopped(a.ctrl)) return;
+    const Geom& g = a.g;
+    const TileCoord tc = tile_of_block(g, TY);
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
+    const int x = tc.x0 + lx, y = tc.y0 + ly;
+    const bool own = x < g.nx && y < g.ny;
+    int hx = -1, hy = -1, hlx = 0, hly = 0;
+    if (tid < TY) { hx = tc.x0 - 1; hy = tc.y0 + tid; hlx = 0; hly = tid + 1; }
+    else if (tid < TY + TX) { hx = tc.x0 + tid - TY; hy = tc.y0 - 1; hlx = tid - TY + 1; hly = 0; }
+    const bool hown = hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
+
+    __shared__ T L[L_N][TY + 1][TX + 1];
+    __shared__ double red[8];
+    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
+
+    // channels of plane z-1 at the own point: P3, Q33, Q13, Q23
+    T zp3 = 0, zq33 = 0, zq13 = 0, zq23 = 0;
+    if (DIM == 3 && tc.z0 > 0 && own) {
+        T p[3], q[6];
+        tgv_dual_at<DIM>(a, x, y, tc.z0 - 1, p, q);
+        zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
+    }
+    double sdd = 0.0, suu = 0.0;
+    for (int z = tc.z0; z < tc.z1; ++z) {
+        T p[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
+        if (own) {
+            tgv_dual_at<DIM>(a, x, y, z, p, q);
+            L[L_P1][ly + 1][lx + 1] = p[0];
+            L[L_P2][ly + 1][lx + 1] = p[1];
+            if (DIM == 3) {
+                L[L_Q11][ly + 1][lx + 1] = q[0];
+                L[L_Q22][ly + 1][lx + 1] = q[1];
+                L[L_Q12][ly + 1][lx + 1] = q[3];
+                L[L_Q13][ly + 1][lx + 1] = q[4];
+                L[L_Q23][ly + 1][lx + 1] = q[5];
+            } else {
+                L[L_Q11][ly + 1][lx + 1] = q[0];
+                L[L_Q22][ly + 1][lx + 1] = q[1];
+                L[L_Q12][ly + 1][lx + 1] = q[2];
+            }
+        }
+        if (hown) {
+            T hp[3], hq[6];
+            tgv_dual_at<DIM>(a, hx, hy, z, hp, hq);
+            L[L_P1][hly][hlx] = hp[0];
+            L[L_P2][hly][hlx] = hp[1];
+            if (DIM == 3) {
+                L[L_Q11][hly][hlx] = hq[0];
+                L[L_Q22][hly][hlx] = hq[1];
+                L[L_Q12][hly][hlx] = hq[3];
+                L[L_Q13][hly][hlx] = hq[4];
+                L[L_Q23][hly][hlx] = hq[5];
+            } else {
+                L[L_Q11][hly][hlx] = hq[0];
+                L[L_Q22][hly][hlx] = hq[1];
+                L[L_Q12][hly][hlx] = hq[2];
+            }
+        }
+        __syncthreads();
+        if (own) {
+            const size_t i = (size_t)z * sxy + (size_t)y * nx + x;
+            // x-predecessor at [ly+1][lx], y-predecessor at [ly][lx+1]
+            const int xl = lx, yl = ly;
+            T dv = bdiff(p[0], L[L_P1][yl + 1][xl], x, g.nx) + bdiff(p[1], L[L_P2][yl][xl + 1], y, g.ny);
+            if (DIM == 3) dv = dv + bdiff(p[2], zp3, z, g.nz);
+            const T ub = a.U[i];
+            const T un = (a.lambda * (ub + a.tau * dv) + a.tau * a.f[i]) / (a.lambda + a.tau);
+            const T ubn = (T)2 * un - ub;
+            if (a.check) {
+                const T df = ubn - ub;
+                sdd += (double)df * (double)df;
+                suu += (double)ubn * (double)ubn;
+            }
+            a.Uout[i] = ubn;
+            T dq[3];
+            if (DIM == 3) {
+                dq[0] = bdiff(q[0], L[L_Q11][yl + 1][xl], x, g.nx) + bdiff(q[3], L[L_Q12][yl][xl + 1], y, g.ny) +
+                        bdiff(q[4], zq13, z, g.nz);
+                dq[1] = bdiff(q[3], L[L_Q12][yl + 1][xl], x, g.nx) + bdiff(q[1], L[L_Q22][yl][xl + 1], y, g.ny) +
+                        bdiff(q[5], zq23, z, g.nz);
+                dq[2] = bdiff(q[4], L[L_Q13][yl + 1][xl], x, g.nx) + bdiff(q[5], L[L_Q23][yl][xl + 1], y, g.ny) +
+                        bdiff(q[2], zq33, z, g.nz);
+            } else {
+                dq[0] = bdiff(q[0], L[L_Q11][yl + 1][xl], x, g.nx) + bdiff(q[2], L[L_Q12][yl][xl + 1], y, g.ny);
+                dq[1] = bdiff(q[2], L[L_Q12][yl + 1][xl], x, g.nx) + bdiff(q[1], L[L_Q22][yl][xl + 1], y, g.ny);
+                dq[2] = 0;
+            }
+            #pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                const T vb = ldT<T>(a.V[c], i);
+                const T vn = vb + a.tau * (p[c] + dq[c]);
+                stT<T>(a.Vout[c], i, (T)2 * vn - vb);
+                stT<T>(a.Pout[c], i, p[c]);
+            }
+            const int nq = DIM == 3 ? 6 : 3;
+            #pragma unroll
+            for (int c = 0; c < nq; ++c) stT<T>(a.Qout[c], i, q[c]);
+        }
+        __syncthreads();
+        zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
+    }
+    if (a.check) {
+        block_sum2(sdd, suu, red);
+        if (tid == 0) {
+            a.partials[2 * tc.lin] = sdd;
+            a.partials[2 * tc.lin + 1] = suu;
+        }
+    }
+}
+
+template <typename T, typename S>
+int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
+              const Opts& opts, hipStream_t s, int* iters_done, float* re) {
+    const int dim = nz == 1 ? 2 : 3;
+    const size_t n = (size_t)nx * ny * nz;
+    *iters_done = 0;
+    *re = 0.f;
+    if (iter <= 0) {   // U = U0 (TGV_core.c:53)
+        CILREG_CK(hipMemcpyAsync(d_out, d_f, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        CILREG_CK(hipStreamSynchronize(s));
+        return 0;
+    }
+    int ty = opts.tile_y, zc = opts.zchunk;
+    auto_tiling(nx, ny, nz, &ty, &zc);
+    const Geom g = make_geom(nx, ny, nz, ty, zc);
+    const int nq = dim == 3 ? 6 : 3;
+
+    Arena ar;
+    size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
+    for (int b = 0; b < 2; ++b) {
+        oU[b] = ar.reserve(n * sizeof(T));
+        for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
+        for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
+        for (int c = 0; c < nq; ++c) oQ[b][c] = ar.reserve(n * sizeof(S));
+    }
+    oCtrl = ar.reserve(sizeof(Ctrl));
+    oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
+    if (ar.allocate() != 0) return -1;
+    launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
+    for (int c = 0; c < dim; ++c) {
+        launch_zero(ar.at(oV[0][c]), n * sizeof(S), s);
+        launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
+    }
+    for (int c = 0; c < nq; ++c) launch_zero(ar.at(oQ[0][c]), n * sizeof(S), s);
+    Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
+    double* partials = (double*)ar.at(oPart);
+    const T tau = (T)pow((double)L2, -0.5);   // TGV_core.c:54-55
+
+    auto step = [&](int k, bool check) {
+        TgvArgs<T, S> a;
+        const int cur = k & 1, nxt = cur ^ 1;
+        a.f = d_f;
+        a.U = (const T*)ar.at(oU[cur]);
+        a.Uout = (T*)ar.at(oU[nxt]);
+        for (int c = 0; c < 3; ++c) {
+            const int cc = c < dim ? c : 0;
+            a.V[c] = (const S*)ar.at(oV[cur][cc]);
+            a.Vout[c] = (S*)ar.at(oV[nxt][cc]);
+            a.P[c] = (const S*)ar.at(oP[cur][cc]);
+            a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
+        }
+        for (int c = 0; c < 6; ++c) {
+            const int cc = c < nq ? c : 0;
+            a.Q[c] = (const S*)ar.at(oQ[cur][cc]);
+            a.Qout[c] = (S*)ar.at(oQ[nxt][cc]);
+        }
+        a.lambda = lambda;
+        a.tau = tau;
+        a.sigma = tau;
+        a.alpha1 = alpha1;
+        a.alpha0 = alpha0;
+        a.check = check;
+        a.partials = partials;
+        a.ctrl = ctrl;
+        a.g = g;
+        dim3 grid(g.nblocks);
+        if (dim == 3) {
+            if (ty == 8) hipLaunchKernelGGL((tgv_iter_kernel<3, 8, T, S>), grid, dim3(TX, 8), 0, s, a);
+            else hipLaunchKernelGGL((tgv_iter_kernel<3, 4, T, S>), grid, dim3(TX, 4), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((tgv_iter_kernel<2, 4, T, S>), grid, dim3(TX, 4), 0, s, a);
+        }
+    };
+    int done = 0;
+    if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
+    *iters_done = done;
+    // the state after iteration `done` (stop) or `iter - 1` lives in buffer (last + 1) & 1
+    const int last = (done < iter) ? done : iter - 1;
+    CILREG_CK(hipMemcpyAsync(d_out, ar.at(oU[(last + 1) & 1]), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    CILREG_CK(hipStreamSynchronize(s));
+    return 0;
+}
+
+template int tgv_solve<float, float>(const float*, float*, float, float, float, int, float, float, int, int, int,
+                                     const Opts&, hipStream_t, int*, float*);
+template int tgv_solve<float, bf16_t>(const float*, float*, float, float, float, int, float, float, int, int, int,
+                                      const Opts&, hipStream_t, int*, float*);
+template int tgv_solve<float, _Float16>(const float*, float*, float, float, float, int, float, float, int, int, int,
+                                        const Opts&, hipStream_t, int*, float*);
+template int tgv_solve<double, double>(const double*, double*, double, double, double, int, double, double, int, int,
+                                       int, const Opts&, hipStream_t, int*, float*);
+
+}  // namespace cilreg

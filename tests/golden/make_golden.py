@@ -1,0 +1,113 @@
+"""Write the golden fixtures from the reference C-OMP library.
+
+Run in a container that holds the reference sources:
+
+    make -C oracle ref            # builds oracle/_ref/libcilreg.so from them
+    python tests/golden/make_golden.py
+
+Each case is a (method, parameters, input) triple; the script runs the
+reference library (oracle/_ref/libcilreg.so, compiled from
+src/Core/regularisers_CPU/*.c) and stores the input, the initial output buffer
+(FGP's first residual reads it), the reference output and the reference
+infovector in ``ref_cases.npz``, with the case table in ``ref_cases.json``.
+
+Inputs are small, non-cubic and seeded: a 2D 37x53 and 3D 11x13x17 piecewise
+constant phantom plus noise, a 3D 24x40x56 volume, and a 96x96 crop of the
+reference's own test image (test/test_imageLena.bin, scaled by 1/255 as in
+test/test_2d_cpu_vs_gpu.py:75-80) with seeded multiplicative noise, run with
+the reference tests' parameters (test_2d_cpu_vs_gpu.py:55-310) at reduced
+iteration counts.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "ccpi-regularisation-toolkit_amd"))
+
+import oracle  # noqa: E402
+from ccpi.supp import phantoms  # noqa: E402
+
+LENA = "/root/reference/test/test_imageLena.bin"
+
+
+def inputs():
+    ins = {}
+    ins["p2d"] = phantoms.noisy(phantoms.shapes_2d((37, 53), 8, seed=11), 0.05, seed=12)
+    ins["p3d"] = phantoms.noisy(phantoms.ellipsoids_3d((11, 13, 17), 8, seed=13), 0.05, seed=14)
+    ins["p3d_big"] = phantoms.noisy(phantoms.ellipsoids_3d((24, 40, 56), 12, seed=15), 0.05, seed=16)
+    im = np.fromfile(LENA, dtype=np.uint8, count=512 * 512).reshape(512, 512).astype(np.float32) / 255
+    crop = im[200:296, 200:296].copy()
+    rng = np.random.Generator(np.random.PCG64(17))
+    noise = phantoms.gaussian_noise(rng, crop.size, 1.0).reshape(crop.shape)
+    ins["lena96"] = (crop + 0.05 * crop * noise).astype(np.float32)
+    # a non-zero initial Output for FGP's first residual (FGP_TV_core.c:66,123)
+    ins["p2d_init"] = (0.5 * ins["p2d"]).astype(np.float32)
+    ins["p3d_init"] = (0.5 * ins["p3d"]).astype(np.float32)
+    return ins
+
+
+def cases():
+    cs = []
+    for inp in ("p2d", "p3d"):
+        cs.append(("rof", inp, {"lambda": 0.02, "iters": 50, "tau": 0.001, "tol": 0.0}))
+        cs.append(("rof", inp, {"lambda": 0.04, "iters": 300, "tau": 0.01, "tol": 2e-3}))
+        for method in (0, 1):
+            for nonneg in (0, 1):
+                cs.append(("fgp", inp, {"lambda": 0.02, "iters": 40, "tol": 0.0, "method": method, "nonneg": nonneg}))
+                cs.append(("fgp", inp, {"lambda": 0.05, "iters": 300, "tol": 1e-3, "method": method, "nonneg": nonneg}))
+                cs.append(("pd", inp, {"lambda": 0.02, "iters": 40, "tol": 0.0, "L": 12.0, "method": method,
+                                       "nonneg": nonneg}))
+                cs.append(("pd", inp, {"lambda": 0.05, "iters": 300, "tol": 1e-3, "L": 8.0, "method": method,
+                                       "nonneg": nonneg}))
+            cs.append(("sb", inp, {"mu": 0.02, "iters": 30, "tol": 0.0, "method": method}))
+            cs.append(("sb", inp, {"mu": 0.05, "iters": 150, "tol": 2e-3, "method": method}))
+        cs.append(("tgv", inp, {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 50, "L2": 12.0, "tol": 0.0}))
+        cs.append(("tgv", inp, {"lambda": 0.05, "alpha1": 1.0, "alpha0": 2.0, "iters": 400, "L2": 12.0, "tol": 1e-3}))
+        cs.append(("fgp", inp, {"lambda": 0.03, "iters": 100, "tol": 1e-3, "method": 0, "nonneg": 0,
+                                "out_init": inp + "_init"}))
+    big = "p3d_big"
+    cs.append(("rof", big, {"lambda": 0.02, "iters": 40, "tau": 0.001, "tol": 0.0}))
+    cs.append(("fgp", big, {"lambda": 0.02, "iters": 40, "tol": 1e-6, "method": 0, "nonneg": 1}))
+    cs.append(("pd", big, {"lambda": 0.02, "iters": 40, "tol": 1e-6, "L": 12.0, "method": 0, "nonneg": 1}))
+    cs.append(("sb", big, {"mu": 0.02, "iters": 20, "tol": 0.0, "method": 0}))
+    cs.append(("tgv", big, {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 40, "L2": 12.0, "tol": 0.0}))
+    # the reference's 2D CPU-vs-GPU test parameters on a crop of its test image
+    lena = "lena96"
+    cs.append(("rof", lena, {"lambda": 0.02, "iters": 1000, "tau": 0.001, "tol": 0.0}))
+    cs.append(("fgp", lena, {"lambda": 0.02, "iters": 400, "tol": 0.0, "method": 0, "nonneg": 0}))
+    cs.append(("pd", lena, {"lambda": 0.02, "iters": 1500, "tol": 0.0, "L": 8.0, "method": 0, "nonneg": 0}))
+    cs.append(("sb", lena, {"mu": 0.02, "iters": 250, "tol": 0.0, "method": 0}))
+    cs.append(("tgv", lena, {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 1000, "L2": 12.0, "tol": 0.0}))
+    return cs
+
+
+def main():
+    ref = oracle.reference()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libcilreg.so is missing: run `make -C oracle ref` first")
+    ins = inputs()
+    arrays = {f"in_{k}": v for k, v in ins.items()}
+    table = []
+    for idx, (method, inp, params) in enumerate(cases()):
+        p = dict(params)
+        init_key = p.pop("out_init", None)
+        out = ins[init_key].copy() if init_key else None
+        y, info = ref.run(method, ins[inp], p, out=out)
+        arrays[f"c{idx}_out"] = y
+        arrays[f"c{idx}_info"] = info
+        table.append({"id": idx, "method": method, "input": inp, "params": p, "out_init": init_key})
+        print(idx, method, inp, p, "info", info.tolist())
+    np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **arrays)
+    with open(os.path.join(HERE, "ref_cases.json"), "w") as fh:
+        json.dump(table, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
