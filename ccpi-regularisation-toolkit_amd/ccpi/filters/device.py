@@ -22,13 +22,32 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 
 class CilregOpts(ctypes.Structure):
     _fields_ = [("check_every", ctypes.c_int), ("dual_precision", ctypes.c_int), ("tile_y", ctypes.c_int),
-                ("zchunk", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("zchunk", ctypes.c_int), ("profile", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
-def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0):
+def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0, profile=False):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
-    return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk))
+    return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk), int(bool(profile)))
+
+
+def last_profile():
+    """(mean_ms, min_ms, count) of the per-iteration kernel time of the last
+    solve on this thread run with ``profile=True``."""
+    lib = require_gpu_lib()
+    lib.cilreg_last_profile.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int)]
+    lib.cilreg_last_profile.restype = ctypes.c_int
+    mean, mn, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    lib.cilreg_last_profile(ctypes.byref(mean), ctypes.byref(mn), ctypes.byref(cnt))
+    return mean.value, mn.value, cnt.value
+
+
+def release_cache():
+    """Free the device workspaces the library keeps cached on the current device."""
+    lib = require_gpu_lib()
+    lib.cilreg_release_cache.restype = ctypes.c_int
+    return lib.cilreg_release_cache()
 
 
 _vp = ctypes.c_void_p

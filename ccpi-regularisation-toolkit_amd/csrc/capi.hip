@@ -64,6 +64,16 @@ extern "C" {
 
 const char* cilreg_last_error(void) { return last_error(); }
 
+int cilreg_last_profile(double* mean_ms, double* min_ms, int* count) {
+    const ProfileStats& ps = thread_profile();
+    if (mean_ms) *mean_ms = ps.mean_ms;
+    if (min_ms) *min_ms = ps.min_ms;
+    if (count) *count = ps.count;
+    return ps.count > 0 ? 0 : -1;
+}
+
+int cilreg_release_cache(void) { return pool_release_idle(); }
+
 // ---------------------------------------------------------------------------
 // reference ABI
 // ---------------------------------------------------------------------------

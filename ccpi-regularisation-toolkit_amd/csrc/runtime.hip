@@ -3,6 +3,7 @@
 #include <stdarg.h>
 #include <string.h>
 #include <mutex>
+#include <stdlib.h>
 
 namespace cilreg {
 
@@ -45,8 +46,104 @@ hipStream_t device_stream() {
     return streams[dev];
 }
 
+// ---------------------------------------------------------------------------
+// caching device allocator
+// ---------------------------------------------------------------------------
+namespace {
+struct PoolBlock {
+    void* p;
+    size_t bytes;
+    int dev;
+    bool busy;
+};
+std::mutex g_pool_mu;
+std::vector<PoolBlock> g_pool;
+
+bool cache_disabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("CILREG_NO_CACHE");
+        v = (e && e[0] && e[0] != '0') ? 1 : 0;
+    }
+    return v == 1;
+}
+
+int free_idle_locked(int dev) {
+    int n = 0;
+    for (size_t i = 0; i < g_pool.size();) {
+        if (!g_pool[i].busy && g_pool[i].dev == dev) {
+            (void)hipFree(g_pool[i].p);
+            g_pool[i] = g_pool.back();
+            g_pool.pop_back();
+            ++n;
+        } else {
+            ++i;
+        }
+    }
+    return n;
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!cache_disabled()) {
+        int best = -1;
+        for (size_t i = 0; i < g_pool.size(); ++i) {
+            const PoolBlock& b = g_pool[i];
+            if (b.busy || b.dev != dev || b.bytes < bytes || b.bytes > 2 * bytes) continue;
+            if (best < 0 || b.bytes < g_pool[best].bytes) best = (int)i;
+        }
+        if (best >= 0) {
+            g_pool[best].busy = true;
+            return g_pool[best].p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        if (free_idle_locked(dev) == 0 || hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    g_pool.push_back(PoolBlock{p, bytes, dev, true});
+    return p;
+}
+
+void pool_free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (size_t i = 0; i < g_pool.size(); ++i) {
+        if (g_pool[i].p == p) {
+            if (cache_disabled()) {
+                (void)hipFree(p);
+                g_pool[i] = g_pool.back();
+                g_pool.pop_back();
+            } else {
+                g_pool[i].busy = false;
+            }
+            return;
+        }
+    }
+}
+
+int pool_release_idle() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    free_idle_locked(dev);
+    return 0;
+}
+
+ProfileStats& thread_profile() {
+    static thread_local ProfileStats ps;
+    return ps;
+}
+
 Arena::~Arena() {
-    if (base_) (void)hipFree(base_);
+    if (base_) pool_free(base_);
 }
 
 size_t Arena::reserve(size_t bytes) {
@@ -57,21 +154,25 @@ size_t Arena::reserve(size_t bytes) {
 
 int Arena::allocate() {
     if (total_ == 0) total_ = 256;
-    hipError_t e = hipMalloc(&base_, total_);
-    if (e != hipSuccess) {
-        base_ = nullptr;
-        set_error("hipMalloc of %zu bytes failed: %s", total_, hipGetErrorString(e));
+    base_ = pool_alloc(total_);
+    if (!base_) {
+        set_error("device allocation of %zu bytes failed", total_);
         return -1;
     }
     return 0;
 }
 
-PinnedCtrl::~PinnedCtrl() {
-    if (h) (void)hipHostFree(h);
-}
+namespace {
+struct PinnedSlot {
+    Ctrl* h = nullptr;
+    ~PinnedSlot() { if (h) (void)hipHostFree(h); }
+};
+}  // namespace
 
 int PinnedCtrl::allocate() {
-    CILREG_CK(hipHostMalloc((void**)&h, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    static thread_local PinnedSlot slot;
+    if (!slot.h) CILREG_CK(hipHostMalloc((void**)&slot.h, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    h = slot.h;
     memset(h, 0, 2 * sizeof(Ctrl));
     return 0;
 }
@@ -181,6 +282,7 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.precision = o->dual_precision;
         r.tile_y = o->tile_y;
         r.zchunk = o->zchunk;
+        r.profile = o->profile;
     }
     return r;
 }

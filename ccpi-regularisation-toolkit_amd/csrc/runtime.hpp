@@ -33,8 +33,15 @@ int select_device(int dev);
 // The library's own stream on the current device (default priority, created once).
 hipStream_t device_stream();
 
-// One device allocation carved into aligned pieces; freed on destruction, so
-// every exit path of a solve releases its memory.
+// Caching device allocator: blocks freed by a finished solve are kept and
+// handed to the next solve that fits (at most 2x its request), so repeated
+// solves do not pay hipMalloc/hipFree.  CILREG_NO_CACHE=1 disables caching.
+void* pool_alloc(size_t bytes);
+void pool_free(void* p);
+int pool_release_idle();
+
+// One device allocation carved into aligned pieces; returned to the pool on
+// destruction, so every exit path of a solve releases its memory.
 class Arena {
 public:
     Arena() = default;
@@ -51,10 +58,10 @@ private:
     size_t total_ = 0;
 };
 
-// Pinned host mirror of the control block (for stop polling).
+// Pinned host mirror of the control block (for stop polling); one cached
+// buffer per host thread.
 class PinnedCtrl {
 public:
-    ~PinnedCtrl();
     int allocate();
     Ctrl* h = nullptr;
 };
@@ -78,7 +85,12 @@ struct Opts {
     int tile_y = 0;        // 0: auto
     int zchunk = 0;        // 0: auto
     int poll = 16;         // host polls the stop flag every `poll` iterations
+    int profile = 0;       // time each iteration with HIP events
 };
+
+// Per-iteration kernel timing of the last profiled solve on this thread.
+struct ProfileStats { double mean_ms = 0, min_ms = 0; int count = 0; };
+ProfileStats& thread_profile();
 Opts resolve_opts(const cilreg_opts* o);
 
 // Default tile height and z-chunk for a volume so the grid has enough blocks
@@ -105,9 +117,17 @@ int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, doub
     }
     int poll_idx = 0;
     int rc = 0;
-    for (int k = 0; k < iter; ++k) {
+    std::vector<hipEvent_t> pev;
+    if (opts.profile) {
+        pev.resize(2 * (size_t)iter, nullptr);
+        for (auto& e : pev)
+            if (hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); rc = -1; break; }
+    }
+    for (int k = 0; k < iter && rc == 0; ++k) {
         const bool check = use_tol && (k % opts.check_every == 0);
+        if (opts.profile) (void)hipEventRecord(pev[2 * k], s);
         step(k, check);
+        if (opts.profile) (void)hipEventRecord(pev[2 * k + 1], s);
         if (check) launch_finalize(d_partials, nblocks, d_ctrl, epsil, k, s);
         if (use_tol && ((k + 1) % opts.poll == 0) && k + 1 < iter) {
             const int slot = poll_idx & 1;
@@ -134,9 +154,29 @@ int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, doub
     e = hipStreamSynchronize(s);
     if (rc == 0 && e != hipSuccess) { set_error("solve failed: %s", hipGetErrorString(e)); rc = -1; }
     for (int i = 0; i < 2; ++i) if (ev[i]) (void)hipEventDestroy(ev[i]);
-    if (rc != 0) return rc;
     Ctrl c;
-    CILREG_CK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    if (rc == 0) {
+        hipError_t ce = hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost);
+        if (ce != hipSuccess) { set_error("control block read failed: %s", hipGetErrorString(ce)); rc = -1; }
+    }
+    if (opts.profile) {
+        ProfileStats& ps = thread_profile();
+        ps = ProfileStats();
+        const int ran = rc == 0 ? (c.stopped ? c.stop_iter + 1 : iter) : 0;
+        double sum = 0, mn = 1e300;
+        int cnt = 0;
+        for (int k = 0; k < ran; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, pev[2 * k], pev[2 * k + 1]) == hipSuccess) {
+                sum += ms;
+                mn = ms < mn ? ms : mn;
+                ++cnt;
+            }
+        }
+        if (cnt) { ps.mean_ms = sum / cnt; ps.min_ms = mn; ps.count = cnt; }
+        for (auto& e : pev) if (e) (void)hipEventDestroy(e);
+    }
+    if (rc != 0) return rc;
     *iters_done = c.stopped ? c.stop_iter : iter;
     *re = c.re;
     return 0;

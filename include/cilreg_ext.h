@@ -37,11 +37,23 @@ typedef struct cilreg_opts {
     int dual_precision;  /* CILREG_FP32 / CILREG_BF16 / CILREG_FP16 storage of dual and auxiliary fields */
     int tile_y;          /* tile height in y (4 or 8); 0 = auto */
     int zchunk;          /* z planes marched per workgroup; 0 = auto */
-    int reserved[4];
+    int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
+    int reserved[3];
 } cilreg_opts;
 
 /* Last error recorded on this host thread ("" if none). */
 CILREG_API const char *cilreg_last_error(void);
+
+/* Per-iteration kernel time of the last solve on this host thread that ran
+ * with opts.profile = 1: mean and minimum in milliseconds over `count`
+ * iterations, measured with HIP events on the solve's stream around the
+ * iteration's main kernel(s) (the stopping-rule finaliser excluded). */
+CILREG_API int cilreg_last_profile(double *mean_ms, double *min_ms, int *count);
+
+/* Frees the device memory the library keeps cached between calls on the
+ * current device (solves reuse freed workspaces instead of calling hipMalloc;
+ * set CILREG_NO_CACHE=1 to allocate and free per call like the reference). */
+CILREG_API int cilreg_release_cache(void);
 
 /* Device-resident solves.  d_input and d_output are device arrays of
  * dimX*dimY*dimZ floats (2D: dimZ = 1); infovector is a host float[2]
