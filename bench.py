@@ -223,7 +223,7 @@ def main():
                    "voxels": n, "iterations_per_step": iters},
         "hbm_GBps_alg": achieved, "roofline": roofline, "cpu_baseline": cpu, "secondary": extras,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=float), flush=True)
 
 
 if __name__ == "__main__":
