@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -63,6 +64,40 @@ template <typename T, typename S>
 __device__ inline T ldT(const S* p, size_t i) { return (T)Store<S>::ld(p, i); }
 template <typename T, typename S>
 __device__ inline void stT(S* p, size_t i, T v) { Store<S>::st(p, i, v); }
+
+// Element access with an index type I.  With I = uint32_t the byte offset is
+// formed in 32 bits (the host guarantees it fits), so loads use the
+// scalar-base + 32-bit vector-offset addressing mode instead of 64-bit
+// address arithmetic per access.
+template <typename I> struct IdxT;
+template <> struct IdxT<uint32_t> {
+    template <typename S> __device__ static inline const S* at(const S* p, uint32_t e) {
+        return (const S*)((const char*)p + (uint32_t)(e * (uint32_t)sizeof(S)));
+    }
+    template <typename S> __device__ static inline S* at(S* p, uint32_t e) {
+        return (S*)((char*)p + (uint32_t)(e * (uint32_t)sizeof(S)));
+    }
+};
+template <> struct IdxT<uint64_t> {
+    template <typename S> __device__ static inline const S* at(const S* p, uint64_t e) { return p + e; }
+    template <typename S> __device__ static inline S* at(S* p, uint64_t e) { return p + e; }
+};
+
+template <typename T, typename I, typename S>
+__device__ inline T ldx(const S* p, I e) {
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        return (T)__uint_as_float(((uint32_t)IdxT<I>::at(p, e)->bits) << 16);
+    } else {
+        return (T)*IdxT<I>::at(p, e);
+    }
+}
+template <typename T, typename I, typename S>
+__device__ inline void stx(S* p, I e, T v) {
+    Store<S>::st(IdxT<I>::at(p, e), 0, v);
+}
+
+// Largest element count addressed with 32-bit byte offsets for element size es.
+inline bool fits_u32(size_t n_elems, size_t es) { return n_elems * es < ((size_t)1 << 31); }
 
 // ---------------------------------------------------------------------------
 // volume geometry and the block -> tile mapping

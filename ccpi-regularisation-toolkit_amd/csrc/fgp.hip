@@ -37,45 +37,57 @@ struct FgpArgs {
     Geom g;
 };
 
-template <int DIM, int TY, typename T, typename S, bool ISO>
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
 __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
     const TileCoord tc = tile_of_block(g, TY);
+    constexpr int NT = TX * TY, NH = TX + TY;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
     const int x = tc.x0 + lx, y = tc.y0 + ly;
     const bool own = x < g.nx && y < g.ny;
-    // +x halo column (tid < TY) and +y halo row (TY <= tid < TY + TX)
-    int hx = -1, hy = -1, hlx = 0, hly = 0;
-    if (tid < TY) { hx = tc.x0 + TX; hy = tc.y0 + tid; hlx = TX; hly = tid; }
-    else if (tid < TY + TX) { hx = tc.x0 + tid - TY; hy = tc.y0 + TY; hlx = tid - TY; hly = TY; }
-    const bool hown = hx >= 0 && hx < g.nx && hy < g.ny;
+    // +x halo column (TY points) and +y halo row (TX points); the halo lanes
+    // alternate between the low and high waves with the block index so
+    // co-resident workgroups spread the extra pass over the SIMDs.
+    const int hbase = (tc.lin & 1) ? NT - NH : 0;
+    const int h = tid - hbase;
+    const bool hlane = h >= 0 && h < NH;
+    int hx, hy, hlx, hly;
+    if (h < TY) { hx = tc.x0 + TX; hy = tc.y0 + h; hlx = TX; hly = h; }
+    else { hx = tc.x0 + h - TY; hy = tc.y0 + TY; hlx = h - TY; hly = TY; }
+    const bool hown = hlane && hx < g.nx && hy < g.ny;
+    // points outside the volume evaluate a valid stand-in and store nothing
+    const int ox = own ? x : 0, oy = own ? y : 0;
+    const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T Ds[2][TY + 1][TX + 1];
     __shared__ double red[8];
 
-    const size_t sxy = (size_t)g.sxy;
-    const size_t nx = (size_t)g.nx;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T lam = a.lambda, mult = a.mult;
-    auto Rv = [&](int c, size_t i) -> T {
-        T p = ldT<T>(a.P[c], i), q = ldT<T>(a.Pp[c], i);
+    auto Rv = [&](int c, I i) -> T {
+        const T p = ldx<T>(a.P[c], i), q = ldx<T>(a.Pp[c], i);
         return p + mult * (p - q);
     };
-    // D at (px, py, z): writes LDS, returns R at the point and takes R3 of the plane below
+    // D at (px, py, z) from R there and at the -x / -y / -z neighbours;
+    // neighbour loads are unconditional (clamped) and masked afterwards.
     auto dval = [&](int px, int py, int z, T r3below, T& r1, T& r2, T& r3) -> T {
-        const size_t i = (size_t)z * sxy + (size_t)py * nx + px;
+        const I i = (I)z * sxy + (I)py * nx + (I)px;
+        const I im = px > 0 ? i - 1 : i, jm = py > 0 ? i - nx : i;
         r1 = Rv(0, i);
         r2 = Rv(1, i);
-        const T b1 = px > 0 ? Rv(0, i - 1) : (T)0;
-        const T b2 = py > 0 ? Rv(1, i - nx) : (T)0;
+        const T b1r = Rv(0, im), b2r = Rv(1, jm);
+        const T fv = ldx<T>(a.f, i);
+        const T b1 = px > 0 ? b1r : (T)0;
+        const T b2 = py > 0 ? b2r : (T)0;
         T d;
         if (DIM == 3) {
             r3 = Rv(2, i);
             const T b3 = z > 0 ? r3below : (T)0;
-            d = a.f[i] - lam * (r1 + r2 + r3 - b1 - b2 - b3);
+            d = fv - lam * (r1 + r2 + r3 - b1 - b2 - b3);
         } else {
             r3 = 0;
-            d = a.f[i] - lam * (r1 + r2 - b1 - b2);
+            d = fv - lam * (r1 + r2 - b1 - b2);
         }
         if (a.nonneg && d < (T)0) d = (T)0;
         return d;
@@ -85,24 +97,28 @@ __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
     T hr3 = 0;                     // R3 at the halo point, plane z
     T r3b = 0, hr3b = 0;           // R3 one plane below the chunk start
     if (DIM == 3 && tc.z0 > 0) {
-        const size_t base = (size_t)(tc.z0 - 1) * sxy;
-        if (own) r3b = Rv(2, base + (size_t)y * nx + x);
-        if (hown) hr3b = Rv(2, base + (size_t)hy * nx + hx);
+        const I base = (I)(tc.z0 - 1) * sxy;
+        r3b = Rv(2, base + (I)oy * nx + (I)ox);
+        if (hlane) hr3b = Rv(2, base + (I)hcy * nx + (I)hcx);
     }
-    if (own) Ds[0][ly][lx] = dval(x, y, tc.z0, r3b, rc1, rc2, rc3);
-    if (hown) { T t1, t2; Ds[0][hly][hlx] = dval(hx, hy, tc.z0, hr3b, t1, t2, hr3); }
+    Ds[0][ly][lx] = dval(ox, oy, tc.z0, r3b, rc1, rc2, rc3);
+    if (hlane) { T t1, t2; const T d = dval(hcx, hcy, tc.z0, hr3b, t1, t2, hr3); if (hown) Ds[0][hly][hlx] = d; }
 
     double sdd = 0.0, suu = 0.0;
     int cur = 0;
     for (int z = tc.z0; z < tc.z1; ++z) {
         T rn1 = 0, rn2 = 0, rn3 = 0, hr3n = 0;
         if (DIM == 3 && z + 1 < g.nz) {
-            if (own) Ds[cur ^ 1][ly][lx] = dval(x, y, z + 1, rc3, rn1, rn2, rn3);
-            if (hown) { T t1, t2; Ds[cur ^ 1][hly][hlx] = dval(hx, hy, z + 1, hr3, t1, t2, hr3n); }
+            Ds[cur ^ 1][ly][lx] = dval(ox, oy, z + 1, rc3, rn1, rn2, rn3);
+            if (hlane) {
+                T t1, t2;
+                const T d = dval(hcx, hcy, z + 1, hr3, t1, t2, hr3n);
+                if (hown) Ds[cur ^ 1][hly][hlx] = d;
+            }
         }
         __syncthreads();
         if (own) {
-            const size_t i = (size_t)z * sxy + (size_t)y * nx + x;
+            const I i = (I)z * sxy + (I)y * nx + (I)x;
             const T d = Ds[cur][ly][lx];
             const T g1 = (x == g.nx - 1) ? (T)0 : d - Ds[cur][ly][lx + 1];
             const T g2 = (y == g.ny - 1) ? (T)0 : d - Ds[cur][ly + 1][lx];
@@ -134,12 +150,12 @@ __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
                     p3 = p3 / a3;
                 }
             }
-            stT<T>(a.Pn[0], i, p1);
-            stT<T>(a.Pn[1], i, p2);
-            if (DIM == 3) stT<T>(a.Pn[2], i, p3);
-            if (a.write_d) a.Dout[i] = d;
+            stx<T>(a.Pn[0], i, p1);
+            stx<T>(a.Pn[1], i, p2);
+            if (DIM == 3) stx<T>(a.Pn[2], i, p3);
+            if (a.write_d) stx<T>(a.Dout, i, d);
             if (a.check) {
-                const T df = d - a.Dprev[i];
+                const T df = d - ldx<T>(a.Dprev, i);
                 sdd += (double)df * (double)df;
                 suu += (double)d * (double)d;
             }
@@ -157,11 +173,21 @@ __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
     }
 }
 
+template <int DIM, int TY, typename T, typename S, typename I>
+static const void* fgp_kernel_ptr(bool iso) {
+    return iso ? (const void*)fgp_iter_kernel<DIM, TY, T, S, true, I> : (const void*)fgp_iter_kernel<DIM, TY, T, S, false, I>;
+}
+
 template <int DIM, int TY, typename T, typename S>
-static void launch_fgp(const FgpArgs<T, S>& a, bool iso, hipStream_t s) {
+static void launch_fgp(const FgpArgs<T, S>& a, bool iso, bool small, hipStream_t s) {
     dim3 block(TX, TY), grid(a.g.nblocks);
-    if (iso) hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, false>), grid, block, 0, s, a);
+    if (small) {
+        if (iso) hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, s, a);
+    } else {
+        if (iso) hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp_iter_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, s, a);
+    }
 }
 
 // Device-resident FGP solve.  d_out receives D of the final iteration (left
@@ -175,10 +201,19 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     *iters_done = 0;
     *re = 0.f;
     if (iter <= 0) return 0;
-    int ty = opts.tile_y, zc = opts.zchunk;
-    auto_tiling(nx, ny, nz, &ty, &zc);
-    const Geom g = make_geom(nx, ny, nz, ty, zc);
     const size_t n = (size_t)nx * ny * nz;
+    const bool small = fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    const int ty = (opts.tile_y == 8 && dim == 3) ? 8 : 4;
+    const bool iso = method == 0;
+    Geom g;
+    if (dim == 2)
+        g = make_geom(nx, ny, nz, 4, 1);
+    else if (ty == 8)
+        g = plan_geom(small ? fgp_kernel_ptr<3, 8, T, S, uint32_t>(iso) : fgp_kernel_ptr<3, 8, T, S, uint64_t>(iso),
+                      TX * 8, nx, ny, nz, 8, opts.zchunk);
+    else
+        g = plan_geom(small ? fgp_kernel_ptr<3, 4, T, S, uint32_t>(iso) : fgp_kernel_ptr<3, 4, T, S, uint64_t>(iso),
+                      TX * 4, nx, ny, nz, 4, opts.zchunk);
 
     Arena ar;
     size_t oP[3][3], oD[2], oCtrl, oPart;
@@ -225,10 +260,10 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
         a.ctrl = ctrl;
         a.g = g;
         if (dim == 3) {
-            if (ty == 8) launch_fgp<3, 8>(a, method == 0, s);
-            else launch_fgp<3, 4>(a, method == 0, s);
+            if (ty == 8) launch_fgp<3, 8>(a, iso, small, s);
+            else launch_fgp<3, 4>(a, iso, small, s);
         } else {
-            launch_fgp<2, 4>(a, method == 0, s);
+            launch_fgp<2, 4>(a, iso, small, s);
         }
         // advance t for the next iteration's R (the reference updates R at the
         // end of iteration k with t_k and t_{k+1})

@@ -287,6 +287,22 @@ Opts resolve_opts(const cilreg_opts* o) {
     return r;
 }
 
+int plan_zchunk(long long tiles, int nz, long long resident) {
+    // a chunk's prologue costs about one plane of dual work
+    const double prologue = 1.0;
+    int best = 1;
+    double best_cost = 1e300;
+    for (int c = 1; c <= nz; ++c) {
+        const int zc = (nz + c - 1) / c;
+        if (c > 1 && (nz + zc - 1) / zc != c) continue;   // same zc as a smaller c
+        const long long blocks = tiles * (long long)((nz + zc - 1) / zc);
+        const long long rounds = (blocks + resident - 1) / resident;
+        const double cost = (double)rounds * (zc + prologue);
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = (nz + zc - 1) / zc; }
+    }
+    return best;
+}
+
 void auto_tiling(int nx, int ny, int nz, int* ty, int* zc) {
     int t = (*ty == 4 || *ty == 8) ? *ty : 4;
     *ty = t;

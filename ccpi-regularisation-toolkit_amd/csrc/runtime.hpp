@@ -97,6 +97,39 @@ Opts resolve_opts(const cilreg_opts* o);
 // to cover 256 CUs several times.
 void auto_tiling(int nx, int ny, int nz, int* ty, int* zc);
 
+// Number of z chunks that fills whole rounds of resident workgroups:
+// minimises rounds * (planes per chunk + prologue cost) for `tiles` xy tiles
+// and `resident` concurrently resident workgroups on the device.
+int plan_zchunk(long long tiles, int nz, long long resident);
+
+// Workgroups of `kernel` (with `threads` threads) resident on the whole
+// current device at once.
+template <class K>
+long long resident_blocks(K kernel, int threads) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    (void)hipGetLastError();
+    return (long long)per_cu * cus;
+}
+
+// Geometry for a marching kernel: explicit zchunk if given, else planned
+// from the kernel's residency.
+template <class K>
+Geom plan_geom(K kernel, int threads, int nx, int ny, int nz, int ty, int zc_hint) {
+    if (zc_hint > 0 || nz <= 1) return make_geom(nx, ny, nz, ty, zc_hint > 0 ? zc_hint : 1);
+    const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + ty - 1) / ty);
+    const long long res = resident_blocks(kernel, threads);
+    if (res <= 0) {
+        int t = ty, zc = 0;
+        auto_tiling(nx, ny, nz, &t, &zc);
+        return make_geom(nx, ny, nz, ty, zc);
+    }
+    const int chunks = plan_zchunk(tiles, nz, res);
+    return make_geom(nx, ny, nz, ty, (nz + chunks - 1) / chunks);
+}
+
 // Drives `iter` iterations on stream s.  step(k, check) launches iteration k
 // (check: whether this iteration evaluates the stopping rule); the driver then
 // launches the finaliser, polls the stop flag every opts.poll iterations

@@ -47,28 +47,38 @@ struct TgvArgs {
 // and the +y neighbour (along y).
 enum { L_P1, L_P2, L_Q11, L_Q22, L_Q12, L_Q13, L_Q23, L_N };
 
-template <int DIM, typename T, typename S>
+// Updated and projected duals at (px, py, z).  Every load is unconditional:
+// neighbour indices are clamped at the last index and the boundary rule is
+// applied with selects, so the ~25 loads issue back to back.
+template <int DIM, typename I, typename T, typename S>
 __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, T p[3], T q[6]) {
     const Geom& g = a.g;
-    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
-    const size_t i = (size_t)z * sxy + (size_t)py * nx + px;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const I i = (I)z * sxy + (I)py * nx + (I)px;
     const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
+    const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
     const T sigma = a.sigma;
-    const T u = a.U[i];
-    T v[3], vx[3], vy[3], vz[3];
+    const T u = ldx<T>(a.U, i), ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
+    const T uz = DIM == 3 ? ldx<T>(a.U, iz) : (T)0;
+    T v[3], vx[3], vy[3], vz[3], po[3], qo[6];
     #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-        v[c] = ldT<T>(a.V[c], i);
-        vx[c] = lx ? v[c] : ldT<T>(a.V[c], i + 1);
-        vy[c] = ly ? v[c] : ldT<T>(a.V[c], i + nx);
-        if (DIM == 3) vz[c] = lz ? v[c] : ldT<T>(a.V[c], i + sxy);
+        v[c] = ldx<T>(a.V[c], i);
+        vx[c] = ldx<T>(a.V[c], ix);
+        vy[c] = ldx<T>(a.V[c], iy);
+        if (DIM == 3) vz[c] = ldx<T>(a.V[c], iz);
+        po[c] = ldx<T>(a.P[c], i);
     }
+    const int nq = DIM == 3 ? 6 : 3;
+    #pragma unroll
+    for (int c = 0; c < nq; ++c) qo[c] = ldx<T>(a.Q[c], i);
+
     // P (TGV_core.c:360-365): forward difference of u_bar minus v_bar
-    p[0] = ldT<T>(a.P[0], i) + sigma * (lx ? -v[0] : (a.U[i + 1] - u) - v[0]);
-    p[1] = ldT<T>(a.P[1], i) + sigma * (ly ? -v[1] : (a.U[i + nx] - u) - v[1]);
+    p[0] = po[0] + sigma * (lx ? -v[0] : (ux - u) - v[0]);
+    p[1] = po[1] + sigma * (ly ? -v[1] : (uy - u) - v[1]);
     T nrm;
     if (DIM == 3) {
-        p[2] = ldT<T>(a.P[2], i) + sigma * (lz ? -v[2] : (a.U[i + sxy] - u) - v[2]);
+        p[2] = po[2] + sigma * (lz ? -v[2] : (uz - u) - v[2]);
         nrm = dsqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / a.alpha1;
     } else {
         p[2] = 0;
@@ -85,17 +95,17 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
     for (int c = 0; c < 3; ++c) d[c][0] = d[c][1] = d[c][2] = 0;
     #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-        if (!lx) d[c][0] = vx[c] - v[c];
-        if (!ly) d[c][1] = vy[c] - v[c];
-        if (DIM == 3 && !lz) d[c][2] = vz[c] - v[c];
+        d[c][0] = lx ? (T)0 : vx[c] - v[c];
+        d[c][1] = ly ? (T)0 : vy[c] - v[c];
+        if (DIM == 3) d[c][2] = lz ? (T)0 : vz[c] - v[c];
     }
     if (DIM == 3) {
-        q[0] = ldT<T>(a.Q[0], i) + sigma * d[0][0];
-        q[1] = ldT<T>(a.Q[1], i) + sigma * d[1][1];
-        q[2] = ldT<T>(a.Q[2], i) + sigma * d[2][2];
-        q[3] = ldT<T>(a.Q[3], i) + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
-        q[4] = ldT<T>(a.Q[4], i) + sigma * ((T)0.5 * (d[2][0] + d[0][2]));
-        q[5] = ldT<T>(a.Q[5], i) + sigma * ((T)0.5 * (d[2][1] + d[1][2]));
+        q[0] = qo[0] + sigma * d[0][0];
+        q[1] = qo[1] + sigma * d[1][1];
+        q[2] = qo[2] + sigma * d[2][2];
+        q[3] = qo[3] + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
+        q[4] = qo[4] + sigma * ((T)0.5 * (d[2][0] + d[0][2]));
+        q[5] = qo[5] + sigma * ((T)0.5 * (d[2][1] + d[1][2]));
         T m = dsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + (T)2 * q[3] * q[3] + (T)2 * q[4] * q[4] +
                     (T)2 * q[5] * q[5]);
         m = m / a.alpha0;
@@ -104,9 +114,9 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
             for (int c = 0; c < 6; ++c) q[c] /= m;
         }
     } else {
-        q[0] = ldT<T>(a.Q[0], i) + sigma * d[0][0];
-        q[1] = ldT<T>(a.Q[1], i) + sigma * d[1][1];
-        q[2] = ldT<T>(a.Q[2], i) + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
+        q[0] = qo[0] + sigma * d[0][0];
+        q[1] = qo[1] + sigma * d[1][1];
+        q[2] = qo[2] + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
         q[3] = q[4] = q[5] = 0;
         T m = dsqrt(q[0] * q[0] + q[1] * q[1] + (T)2 * q[2] * q[2]);
         m = m / a.alpha0;
@@ -127,105 +137,112 @@ __device__ inline T bdiff(T own, T pred, int pos, int len) {
     return own - pred;
 }
 
-template <int DIM, int TY, typename T, typename S>
+template <int DIM, int TY, typename T, typename S, typename I>
 __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
     const TileCoord tc = tile_of_block(g, TY);
+    constexpr int NT = TX * TY, NH = TX + TY;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
     const int x = tc.x0 + lx, y = tc.y0 + ly;
     const bool own = x < g.nx && y < g.ny;
-    int hx = -1, hy = -1, hlx = 0, hly = 0;
-    if (tid < TY) { hx = tc.x0 - 1; hy = tc.y0 + tid; hlx = 0; hly = tid + 1; }
-    else if (tid < TY + TX) { hx = tc.x0 + tid - TY; hy = tc.y0 - 1; hlx = tid - TY + 1; hly = 0; }
-    const bool hown = hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
+    // -x halo column (TY points) and -y halo row (TX points).  The lanes doing
+    // the halo pass alternate between the low and the high waves of the block
+    // with the block index, so neighbouring workgroups on a CU load different
+    // SIMDs with the extra pass.
+    const int hbase = (tc.lin & 1) ? NT - NH : 0;
+    const int h = tid - hbase;
+    const bool hlane = h >= 0 && h < NH;
+    int hx, hy, hlx, hly;
+    if (h < TY) { hx = tc.x0 - 1; hy = tc.y0 + h; hlx = 0; hly = h + 1; }
+    else { hx = tc.x0 + h - TY; hy = tc.y0 - 1; hlx = h - TY + 1; hly = 0; }
+    const bool hown = hlane && hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
+    // points outside the volume compute on a valid stand-in and store nothing
+    const int ox = own ? x : 0, oy = own ? y : 0;
+    const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T L[L_N][TY + 1][TX + 1];
     __shared__ double red[8];
-    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
 
     // channels of plane z-1 at the own point: P3, Q33, Q13, Q23
     T zp3 = 0, zq33 = 0, zq13 = 0, zq23 = 0;
-    if (DIM == 3 && tc.z0 > 0 && own) {
+    if (DIM == 3 && tc.z0 > 0) {
         T p[3], q[6];
-        tgv_dual_at<DIM>(a, x, y, tc.z0 - 1, p, q);
+        tgv_dual_at<DIM, I>(a, ox, oy, tc.z0 - 1, p, q);
         zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
     }
     double sdd = 0.0, suu = 0.0;
     for (int z = tc.z0; z < tc.z1; ++z) {
-        T p[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
-        if (own) {
-            tgv_dual_at<DIM>(a, x, y, z, p, q);
-            L[L_P1][ly + 1][lx + 1] = p[0];
-            L[L_P2][ly + 1][lx + 1] = p[1];
+        T p[3], q[6];
+        tgv_dual_at<DIM, I>(a, ox, oy, z, p, q);
+        {
+            const int r = ly + 1, c = lx + 1;
+            L[L_P1][r][c] = p[0];
+            L[L_P2][r][c] = p[1];
+            L[L_Q11][r][c] = q[0];
+            L[L_Q22][r][c] = q[1];
             if (DIM == 3) {
-                L[L_Q11][ly + 1][lx + 1] = q[0];
-                L[L_Q22][ly + 1][lx + 1] = q[1];
-                L[L_Q12][ly + 1][lx + 1] = q[3];
-                L[L_Q13][ly + 1][lx + 1] = q[4];
-                L[L_Q23][ly + 1][lx + 1] = q[5];
+                L[L_Q12][r][c] = q[3];
+                L[L_Q13][r][c] = q[4];
+                L[L_Q23][r][c] = q[5];
             } else {
-                L[L_Q11][ly + 1][lx + 1] = q[0];
-                L[L_Q22][ly + 1][lx + 1] = q[1];
-                L[L_Q12][ly + 1][lx + 1] = q[2];
+                L[L_Q12][r][c] = q[2];
             }
         }
-        if (hown) {
+        if (hlane) {
             T hp[3], hq[6];
-            tgv_dual_at<DIM>(a, hx, hy, z, hp, hq);
+            tgv_dual_at<DIM, I>(a, hcx, hcy, z, hp, hq);
             L[L_P1][hly][hlx] = hp[0];
             L[L_P2][hly][hlx] = hp[1];
+            L[L_Q11][hly][hlx] = hq[0];
+            L[L_Q22][hly][hlx] = hq[1];
             if (DIM == 3) {
-                L[L_Q11][hly][hlx] = hq[0];
-                L[L_Q22][hly][hlx] = hq[1];
                 L[L_Q12][hly][hlx] = hq[3];
                 L[L_Q13][hly][hlx] = hq[4];
                 L[L_Q23][hly][hlx] = hq[5];
             } else {
-                L[L_Q11][hly][hlx] = hq[0];
-                L[L_Q22][hly][hlx] = hq[1];
                 L[L_Q12][hly][hlx] = hq[2];
             }
         }
         __syncthreads();
         if (own) {
-            const size_t i = (size_t)z * sxy + (size_t)y * nx + x;
+            const I i = (I)z * sxy + (I)y * nx + (I)x;
             // x-predecessor at [ly+1][lx], y-predecessor at [ly][lx+1]
-            const int xl = lx, yl = ly;
-            T dv = bdiff(p[0], L[L_P1][yl + 1][xl], x, g.nx) + bdiff(p[1], L[L_P2][yl][xl + 1], y, g.ny);
+            T dv = bdiff(p[0], L[L_P1][ly + 1][lx], x, g.nx) + bdiff(p[1], L[L_P2][ly][lx + 1], y, g.ny);
             if (DIM == 3) dv = dv + bdiff(p[2], zp3, z, g.nz);
-            const T ub = a.U[i];
-            const T un = (a.lambda * (ub + a.tau * dv) + a.tau * a.f[i]) / (a.lambda + a.tau);
+            const T ub = ldx<T>(a.U, i);
+            const T un = (a.lambda * (ub + a.tau * dv) + a.tau * ldx<T>(a.f, i)) / (a.lambda + a.tau);
             const T ubn = (T)2 * un - ub;
             if (a.check) {
                 const T df = ubn - ub;
                 sdd += (double)df * (double)df;
                 suu += (double)ubn * (double)ubn;
             }
-            a.Uout[i] = ubn;
+            stx<T>(a.Uout, i, ubn);
             T dq[3];
             if (DIM == 3) {
-                dq[0] = bdiff(q[0], L[L_Q11][yl + 1][xl], x, g.nx) + bdiff(q[3], L[L_Q12][yl][xl + 1], y, g.ny) +
+                dq[0] = bdiff(q[0], L[L_Q11][ly + 1][lx], x, g.nx) + bdiff(q[3], L[L_Q12][ly][lx + 1], y, g.ny) +
                         bdiff(q[4], zq13, z, g.nz);
-                dq[1] = bdiff(q[3], L[L_Q12][yl + 1][xl], x, g.nx) + bdiff(q[1], L[L_Q22][yl][xl + 1], y, g.ny) +
+                dq[1] = bdiff(q[3], L[L_Q12][ly + 1][lx], x, g.nx) + bdiff(q[1], L[L_Q22][ly][lx + 1], y, g.ny) +
                         bdiff(q[5], zq23, z, g.nz);
-                dq[2] = bdiff(q[4], L[L_Q13][yl + 1][xl], x, g.nx) + bdiff(q[5], L[L_Q23][yl][xl + 1], y, g.ny) +
+                dq[2] = bdiff(q[4], L[L_Q13][ly + 1][lx], x, g.nx) + bdiff(q[5], L[L_Q23][ly][lx + 1], y, g.ny) +
                         bdiff(q[2], zq33, z, g.nz);
             } else {
-                dq[0] = bdiff(q[0], L[L_Q11][yl + 1][xl], x, g.nx) + bdiff(q[2], L[L_Q12][yl][xl + 1], y, g.ny);
-                dq[1] = bdiff(q[2], L[L_Q12][yl + 1][xl], x, g.nx) + bdiff(q[1], L[L_Q22][yl][xl + 1], y, g.ny);
+                dq[0] = bdiff(q[0], L[L_Q11][ly + 1][lx], x, g.nx) + bdiff(q[2], L[L_Q12][ly][lx + 1], y, g.ny);
+                dq[1] = bdiff(q[2], L[L_Q12][ly + 1][lx], x, g.nx) + bdiff(q[1], L[L_Q22][ly][lx + 1], y, g.ny);
                 dq[2] = 0;
             }
             #pragma unroll
             for (int c = 0; c < DIM; ++c) {
-                const T vb = ldT<T>(a.V[c], i);
+                const T vb = ldx<T>(a.V[c], i);
                 const T vn = vb + a.tau * (p[c] + dq[c]);
-                stT<T>(a.Vout[c], i, (T)2 * vn - vb);
-                stT<T>(a.Pout[c], i, p[c]);
+                stx<T>(a.Vout[c], i, (T)2 * vn - vb);
+                stx<T>(a.Pout[c], i, p[c]);
             }
             const int nq = DIM == 3 ? 6 : 3;
             #pragma unroll
-            for (int c = 0; c < nq; ++c) stT<T>(a.Qout[c], i, q[c]);
+            for (int c = 0; c < nq; ++c) stx<T>(a.Qout[c], i, q[c]);
         }
         __syncthreads();
         zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
@@ -237,6 +254,13 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
             a.partials[2 * tc.lin + 1] = suu;
         }
     }
+}
+
+template <int DIM, int TY, typename T, typename S>
+static void launch_tgv(const TgvArgs<T, S>& a, bool small, hipStream_t s) {
+    dim3 grid(a.g.nblocks), block(TX, TY);
+    if (small) hipLaunchKernelGGL((tgv_iter_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((tgv_iter_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, s, a);
 }
 
 template <typename T, typename S>
@@ -251,10 +275,18 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         CILREG_CK(hipStreamSynchronize(s));
         return 0;
     }
-    int ty = opts.tile_y, zc = opts.zchunk;
-    auto_tiling(nx, ny, nz, &ty, &zc);
-    const Geom g = make_geom(nx, ny, nz, ty, zc);
     const int nq = dim == 3 ? 6 : 3;
+    const bool small = fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    int ty = opts.tile_y == 8 && dim == 3 ? 8 : 4;
+    Geom g;
+    if (dim == 2)
+        g = make_geom(nx, ny, nz, 4, 1);
+    else if (ty == 8)
+        g = plan_geom(small ? (const void*)tgv_iter_kernel<3, 8, T, S, uint32_t> : (const void*)tgv_iter_kernel<3, 8, T, S, uint64_t>,
+                      TX * 8, nx, ny, nz, 8, opts.zchunk);
+    else
+        g = plan_geom(small ? (const void*)tgv_iter_kernel<3, 4, T, S, uint32_t> : (const void*)tgv_iter_kernel<3, 4, T, S, uint64_t>,
+                      TX * 4, nx, ny, nz, 4, opts.zchunk);
 
     Arena ar;
     size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
@@ -304,12 +336,11 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         a.partials = partials;
         a.ctrl = ctrl;
         a.g = g;
-        dim3 grid(g.nblocks);
         if (dim == 3) {
-            if (ty == 8) hipLaunchKernelGGL((tgv_iter_kernel<3, 8, T, S>), grid, dim3(TX, 8), 0, s, a);
-            else hipLaunchKernelGGL((tgv_iter_kernel<3, 4, T, S>), grid, dim3(TX, 4), 0, s, a);
+            if (ty == 8) launch_tgv<3, 8>(a, small, s);
+            else launch_tgv<3, 4>(a, small, s);
         } else {
-            hipLaunchKernelGGL((tgv_iter_kernel<2, 4, T, S>), grid, dim3(TX, 4), 0, s, a);
+            launch_tgv<2, 4>(a, small, s);
         }
     };
     int done = 0;
