@@ -201,7 +201,7 @@ def main():
     traffic = read_traffic(n)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": "fgp_iter_kernel<3,4,float,float,ISO>", "kernel_ms_mean": mean_ms,
+                "kernel": "fgp_iter_kernel<3,8,float,float,iso,uint32>", "kernel_ms_mean": mean_ms,
                 "kernel_ms_min": min_ms, "launches_timed": cnt, "alg_bytes_per_voxel": 48}
 
     cpu = None

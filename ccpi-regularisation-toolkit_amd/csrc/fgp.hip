@@ -202,8 +202,8 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     *re = 0.f;
     if (iter <= 0) return 0;
     const size_t n = (size_t)nx * ny * nz;
-    const bool small = fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
-    const int ty = (opts.tile_y == 8 && dim == 3) ? 8 : 4;
+    const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
     Geom g;
     if (dim == 2)

@@ -37,39 +37,49 @@ struct PdArgs {
     Geom g;
 };
 
-template <int DIM, int TY, typename T, typename S, bool ISO>
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
 __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
     const TileCoord tc = tile_of_block(g, TY);
+    constexpr int NT = TX * TY, NH = TX + TY;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
     const int x = tc.x0 + lx, y = tc.y0 + ly;
     const bool own = x < g.nx && y < g.ny;
-    // -x halo column (tid < TY) and -y halo row (TY <= tid < TY + TX)
-    int hx = -1, hy = -1, hlx = 0, hly = 0;
-    if (tid < TY) { hx = tc.x0 - 1; hy = tc.y0 + tid; hlx = 0; hly = tid + 1; }
-    else if (tid < TY + TX) { hx = tc.x0 + tid - TY; hy = tc.y0 - 1; hlx = tid - TY + 1; hly = 0; }
-    const bool hown = hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
+    // -x halo column (TY points) and -y halo row (TX points), on the low or
+    // high waves depending on the block index (spreads the extra pass over
+    // the SIMDs of a CU)
+    const int hbase = (tc.lin & 1) ? NT - NH : 0;
+    const int h = tid - hbase;
+    const bool hlane = h >= 0 && h < NH;
+    int hx, hy, hlx, hly;
+    if (h < TY) { hx = tc.x0 - 1; hy = tc.y0 + h; hlx = 0; hly = h + 1; }
+    else { hx = tc.x0 + h - TY; hy = tc.y0 - 1; hlx = h - TY + 1; hly = 0; }
+    const bool hown = hlane && hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
+    const int ox = own ? x : 0, oy = own ? y : 0;
+    const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
-    __shared__ T Pl[DIM][TY + 1][TX + 1];
+    __shared__ T Pl[2][TY + 1][TX + 1];
     __shared__ double red[8];
 
-    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T sigma = a.sigma;
 
-    // updated, projected dual at (px, py, z)
+    // updated, projected dual at (px, py, z); the forward neighbour is
+    // mirrored to the backward one at the last index (PD_TV_core.c:221-226)
     auto dual = [&](int px, int py, int z, T& q1, T& q2, T& q3) {
-        const size_t i = (size_t)z * sxy + (size_t)py * nx + px;
-        const T u = a.U[i];
-        T d1 = 0, d2 = 0, d3 = 0;
-        if (g.nx > 1) d1 = (px == g.nx - 1) ? a.U[i - 1] - u : a.U[i + 1] - u;
-        if (g.ny > 1) d2 = (py == g.ny - 1) ? a.U[i - nx] - u : a.U[i + nx] - u;
-        q1 = ldT<T>(a.P[0], i) + sigma * d1;
-        q2 = ldT<T>(a.P[1], i) + sigma * d2;
+        const I i = (I)z * sxy + (I)py * nx + (I)px;
+        const I jx = g.nx == 1 ? i : (px == g.nx - 1 ? i - 1 : i + 1);
+        const I jy = g.ny == 1 ? i : (py == g.ny - 1 ? i - nx : i + nx);
+        const T u = ldx<T>(a.U, i), ux = ldx<T>(a.U, jx), uy = ldx<T>(a.U, jy);
+        const T p1 = ldx<T>(a.P[0], i), p2 = ldx<T>(a.P[1], i);
+        q1 = p1 + sigma * (ux - u);
+        q2 = p2 + sigma * (uy - u);
         q3 = 0;
         if (DIM == 3) {
-            if (g.nz > 1) d3 = (z == g.nz - 1) ? a.U[i - sxy] - u : a.U[i + sxy] - u;
-            q3 = ldT<T>(a.P[2], i) + sigma * d3;
+            const I jz = g.nz == 1 ? i : (z == g.nz - 1 ? i - sxy : i + sxy);
+            const T uz = ldx<T>(a.U, jz), p3 = ldx<T>(a.P[2], i);
+            q3 = p3 + sigma * (uz - u);
         }
         if (ISO) {
             T n2 = q1 * q1 + q2 * q2;
@@ -95,28 +105,26 @@ __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
     };
 
     T p3below = 0;   // updated P3 at the own point, plane z-1
-    if (DIM == 3 && tc.z0 > 0 && own) {
+    if (DIM == 3 && tc.z0 > 0) {
         T q1, q2;
-        dual(x, y, tc.z0 - 1, q1, q2, p3below);
+        dual(ox, oy, tc.z0 - 1, q1, q2, p3below);
     }
     double sdd = 0.0, suu = 0.0;
     for (int z = tc.z0; z < tc.z1; ++z) {
-        T q1 = 0, q2 = 0, q3 = 0;
-        if (own) {
-            dual(x, y, z, q1, q2, q3);
-            Pl[0][ly + 1][lx + 1] = q1;
-            Pl[1][ly + 1][lx + 1] = q2;
-        }
-        if (hown) {
+        T q1, q2, q3;
+        dual(ox, oy, z, q1, q2, q3);
+        Pl[0][ly + 1][lx + 1] = q1;
+        Pl[1][ly + 1][lx + 1] = q2;
+        if (hlane) {
             T h1, h2, h3;
-            dual(hx, hy, z, h1, h2, h3);
+            dual(hcx, hcy, z, h1, h2, h3);
             Pl[0][hly][hlx] = h1;
             Pl[1][hly][hlx] = h2;
         }
         __syncthreads();
         if (own) {
-            const size_t i = (size_t)z * sxy + (size_t)y * nx + x;
-            T u0 = a.U[i];
+            const I i = (I)z * sxy + (I)y * nx + (I)x;
+            T u0 = ldx<T>(a.U, i);
             if (a.nonneg && u0 < (T)0) u0 = 0;
             const T v1 = (x == 0) ? -q1 : -(q1 - Pl[0][ly + 1][lx]);
             const T v2 = (y == 0) ? -q2 : -(q2 - Pl[1][ly][lx + 1]);
@@ -125,17 +133,17 @@ __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
                 const T v3 = (z == 0) ? -q3 : -(q3 - p3below);
                 dv = dv + v3;
             }
-            const T un = Mixed<T>::pd_div(u0 - a.tau * dv + a.lt * a.f[i], a.lt);
+            const T un = Mixed<T>::pd_div(u0 - a.tau * dv + a.lt * ldx<T>(a.f, i), a.lt);
             if (a.check) {
                 const T df = un - u0;
                 sdd += (double)df * (double)df;
                 suu += (double)un * (double)un;
-                a.Ubrk[i] = un;
+                stx<T>(a.Ubrk, i, un);
             }
-            a.Uout[i] = un + a.theta * (un - u0);
-            stT<T>(a.Pout[0], i, q1);
-            stT<T>(a.Pout[1], i, q2);
-            if (DIM == 3) stT<T>(a.Pout[2], i, q3);
+            stx<T>(a.Uout, i, un + a.theta * (un - u0));
+            stx<T>(a.Pout[0], i, q1);
+            stx<T>(a.Pout[1], i, q2);
+            if (DIM == 3) stx<T>(a.Pout[2], i, q3);
         }
         __syncthreads();
         p3below = q3;
@@ -149,11 +157,21 @@ __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
     }
 }
 
+template <int DIM, int TY, typename T, typename S, typename I>
+static const void* pd_kernel_ptr(bool iso) {
+    return iso ? (const void*)pd_iter_kernel<DIM, TY, T, S, true, I> : (const void*)pd_iter_kernel<DIM, TY, T, S, false, I>;
+}
+
 template <int DIM, int TY, typename T, typename S>
-static void launch_pd(const PdArgs<T, S>& a, bool iso, hipStream_t s) {
+static void launch_pd(const PdArgs<T, S>& a, bool iso, bool small, hipStream_t s) {
     dim3 block(TX, TY), grid(a.g.nblocks);
-    if (iso) hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, false>), grid, block, 0, s, a);
+    if (small) {
+        if (iso) hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, s, a);
+    } else {
+        if (iso) hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((pd_iter_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, s, a);
+    }
 }
 
 template <typename T, typename S>
@@ -169,9 +187,18 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
         CILREG_CK(hipStreamSynchronize(s));
         return 0;
     }
-    int ty = opts.tile_y, zc = opts.zchunk;
-    auto_tiling(nx, ny, nz, &ty, &zc);
-    const Geom g = make_geom(nx, ny, nz, ty, zc);
+    const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
+    const bool iso = method == 0;
+    Geom g;
+    if (dim == 2)
+        g = make_geom(nx, ny, nz, 4, 1);
+    else if (ty == 8)
+        g = plan_geom(small ? pd_kernel_ptr<3, 8, T, S, uint32_t>(iso) : pd_kernel_ptr<3, 8, T, S, uint64_t>(iso),
+                      TX * 8, nx, ny, nz, 8, opts.zchunk);
+    else
+        g = plan_geom(small ? pd_kernel_ptr<3, 4, T, S, uint32_t>(iso) : pd_kernel_ptr<3, 4, T, S, uint64_t>(iso),
+                      TX * 4, nx, ny, nz, 4, opts.zchunk);
 
     Arena ar;
     size_t oU[2], oP[2][3], oB, oCtrl, oPart;
@@ -213,10 +240,10 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
         a.ctrl = ctrl;
         a.g = g;
         if (dim == 3) {
-            if (ty == 8) launch_pd<3, 8>(a, method == 0, s);
-            else launch_pd<3, 4>(a, method == 0, s);
+            if (ty == 8) launch_pd<3, 8>(a, iso, small, s);
+            else launch_pd<3, 4>(a, iso, small, s);
         } else {
-            launch_pd<2, 4>(a, method == 0, s);
+            launch_pd<2, 4>(a, iso, small, s);
         }
     };
     int done = 0;

@@ -283,6 +283,7 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.tile_y = o->tile_y;
         r.zchunk = o->zchunk;
         r.profile = o->profile;
+        r.wide_index = o->wide_index;
     }
     return r;
 }

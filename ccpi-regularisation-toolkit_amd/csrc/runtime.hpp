@@ -86,6 +86,7 @@ struct Opts {
     int zchunk = 0;        // 0: auto
     int poll = 16;         // host polls the stop flag every `poll` iterations
     int profile = 0;       // time each iteration with HIP events
+    int wide_index = 0;    // force 64-bit element indexing
 };
 
 // Per-iteration kernel timing of the last profiled solve on this thread.

@@ -40,34 +40,36 @@ __device__ inline int sb_p(int i, int n) { return i + 1 == n ? (i - 1 >= 0 ? i -
 __device__ inline int sb_m(int i, int n) { return i - 1 < 0 ? (i + 1 < n ? i + 1 : 0) : i - 1; }
 
 // one Jacobi sweep value at (px, py, z) from a.Uin
-template <int DIM, typename T>
+template <int DIM, typename I, typename T>
 __device__ inline T sb_sweep_at(const SbArgs<T>& a, int px, int py, int z) {
     const Geom& g = a.g;
-    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
-    const size_t zo = (size_t)z * sxy;
-    const size_t c = zo + (size_t)py * nx + px;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const I zo = (I)z * sxy;
+    const I c = zo + (I)py * nx + (I)px;
     const int ip = sb_p(px, g.nx), im = sb_m(px, g.nx), jp = sb_p(py, g.ny), jm = sb_m(py, g.ny);
-    const size_t cim = zo + (size_t)py * nx + im, cjm = zo + (size_t)jm * nx + px;
+    const I cim = zo + (I)py * nx + (I)im, cjm = zo + (I)jm * nx + (I)px;
+    const I cip = zo + (I)py * nx + (I)ip, cjp = zo + (I)jp * nx + (I)px;
     const T* U = a.Uin;
+    auto L = [](const T* p, I e) { return ldx<T>(p, e); };
     T acc;
     if (DIM == 2) {
-        acc = a.D[0][cim] - a.D[0][c] + a.D[1][cjm] - a.D[1][c] - a.B[0][cim] + a.B[0][c] - a.B[1][cjm] + a.B[1][c];
-        acc += U[zo + (size_t)py * nx + ip] + U[cim] + U[zo + (size_t)jp * nx + px] + U[cjm];
+        acc = L(a.D[0], cim) - L(a.D[0], c) + L(a.D[1], cjm) - L(a.D[1], c) - L(a.B[0], cim) + L(a.B[0], c) -
+              L(a.B[1], cjm) + L(a.B[1], c);
+        acc += L(U, cip) + L(U, cim) + L(U, cjp) + L(U, cjm);
     } else {
         const int kp = sb_p(z, g.nz), km = sb_m(z, g.nz);
-        const size_t ckm = (size_t)km * sxy + (size_t)py * nx + px;
-        const T dsum = a.D[0][cim] - a.D[0][c] + a.D[1][cjm] - a.D[1][c] + a.D[2][ckm] - a.D[2][c];
-        const T bsum = -a.B[0][cim] + a.B[0][c] - a.B[1][cjm] + a.B[1][c] - a.B[2][ckm] + a.B[2][c];
+        const I ckm = (I)km * sxy + (I)py * nx + (I)px, ckp = (I)kp * sxy + (I)py * nx + (I)px;
+        const T dsum = L(a.D[0], cim) - L(a.D[0], c) + L(a.D[1], cjm) - L(a.D[1], c) + L(a.D[2], ckm) - L(a.D[2], c);
+        const T bsum = -L(a.B[0], cim) + L(a.B[0], c) - L(a.B[1], cjm) + L(a.B[1], c) - L(a.B[2], ckm) + L(a.B[2], c);
         acc = dsum + bsum;
-        acc += U[zo + (size_t)py * nx + ip] + U[cim] + U[zo + (size_t)jp * nx + px] + U[cjm] +
-               U[(size_t)kp * sxy + (size_t)py * nx + px] + U[ckm];
+        acc += L(U, cip) + L(U, cim) + L(U, cjp) + L(U, cjm) + L(U, ckp) + L(U, ckm);
     }
     acc *= a.lam;
-    acc += a.mu * a.f[c];
+    acc += a.mu * L(a.f, c);
     return a.nc * acc;
 }
 
-template <int DIM, int TY, typename T>
+template <int DIM, int TY, typename T, typename I>
 __global__ __launch_bounds__(TX * TY) void sb_sweep_kernel(SbArgs<T> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
@@ -75,52 +77,62 @@ __global__ __launch_bounds__(TX * TY) void sb_sweep_kernel(SbArgs<T> a) {
     const int x = tc.x0 + (int)threadIdx.x, y = tc.y0 + (int)threadIdx.y;
     if (x >= g.nx || y >= g.ny) return;
     for (int z = tc.z0; z < tc.z1; ++z)
-        a.Uout[(size_t)z * g.sxy + (size_t)y * g.nx + x] = sb_sweep_at<DIM>(a, x, y, z);
+        stx<T>(a.Uout, (I)z * (I)g.sxy + (I)y * (I)g.nx + (I)x, sb_sweep_at<DIM, I>(a, x, y, z));
 }
 
-template <int DIM, int TY, typename T, bool ANISO>
+template <int DIM, int TY, typename T, bool ANISO, typename I>
 __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
     const TileCoord tc = tile_of_block(g, TY);
+    constexpr int NT = TX * TY, NH = TX + TY;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
     const int x = tc.x0 + lx, y = tc.y0 + ly;
     const bool own = x < g.nx && y < g.ny;
-    int hx = -1, hy = -1, hlx = 0, hly = 0;
-    if (tid < TY) { hx = tc.x0 + TX; hy = tc.y0 + tid; hlx = TX; hly = tid; }
-    else if (tid < TY + TX) { hx = tc.x0 + tid - TY; hy = tc.y0 + TY; hlx = tid - TY; hly = TY; }
-    const bool hown = hx >= 0 && hx < g.nx && hy < g.ny;
+    // +x halo column and +y halo row, on the low or high waves by block index
+    const int hbase = (tc.lin & 1) ? NT - NH : 0;
+    const int h = tid - hbase;
+    const bool hlane = h >= 0 && h < NH;
+    int hx, hy, hlx, hly;
+    if (h < TY) { hx = tc.x0 + TX; hy = tc.y0 + h; hlx = TX; hly = h; }
+    else { hx = tc.x0 + h - TY; hy = tc.y0 + TY; hlx = h - TY; hly = TY; }
+    const bool hown = hlane && hx < g.nx && hy < g.ny;
+    const int ox = own ? x : 0, oy = own ? y : 0;
+    const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T Ul[2][TY + 1][TX + 1];
     __shared__ double red[8];
-    const size_t sxy = (size_t)g.sxy, nx = (size_t)g.nx;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
 
     T below = 0;   // U2 at the own point, plane z-1
-    if (DIM == 3 && own && tc.z0 > 0) below = sb_sweep_at<DIM>(a, x, y, tc.z0 - 1);
-    if (own) Ul[0][ly][lx] = sb_sweep_at<DIM>(a, x, y, tc.z0);
-    if (hown) Ul[0][hly][hlx] = sb_sweep_at<DIM>(a, hx, hy, tc.z0);
+    if (DIM == 3 && tc.z0 > 0) below = sb_sweep_at<DIM, I>(a, ox, oy, tc.z0 - 1);
+    Ul[0][ly][lx] = sb_sweep_at<DIM, I>(a, ox, oy, tc.z0);
+    if (hlane) { const T v = sb_sweep_at<DIM, I>(a, hcx, hcy, tc.z0); if (hown) Ul[0][hly][hlx] = v; }
 
     double sdd = 0.0, suu = 0.0;
     int cur = 0;
     for (int z = tc.z0; z < tc.z1; ++z) {
         if (DIM == 3 && z + 1 < g.nz) {
-            if (own) Ul[cur ^ 1][ly][lx] = sb_sweep_at<DIM>(a, x, y, z + 1);
-            if (hown) Ul[cur ^ 1][hly][hlx] = sb_sweep_at<DIM>(a, hx, hy, z + 1);
+            Ul[cur ^ 1][ly][lx] = sb_sweep_at<DIM, I>(a, ox, oy, z + 1);
+            if (hlane) {
+                const T v = sb_sweep_at<DIM, I>(a, hcx, hcy, z + 1);
+                if (hown) Ul[cur ^ 1][hly][hlx] = v;
+            }
         }
         __syncthreads();
         T u2 = 0;
         if (own) {
-            const size_t i = (size_t)z * sxy + (size_t)y * nx + x;
+            const I i = (I)z * sxy + (I)y * nx + (I)x;
             u2 = Ul[cur][ly][lx];
             // mirrored forward neighbours: +1, or -1 at the last index
             T ux, uy;
             if (x + 1 < g.nx) ux = Ul[cur][ly][lx + 1];
             else if (g.nx == 1) ux = u2;
-            else ux = (lx > 0) ? Ul[cur][ly][lx - 1] : sb_sweep_at<DIM>(a, x - 1, y, z);
+            else ux = (lx > 0) ? Ul[cur][ly][lx - 1] : sb_sweep_at<DIM, I>(a, x - 1, y, z);
             if (y + 1 < g.ny) uy = Ul[cur][ly + 1][lx];
             else if (g.ny == 1) uy = u2;
-            else uy = (ly > 0) ? Ul[cur][ly - 1][lx] : sb_sweep_at<DIM>(a, x, y - 1, z);
-            T gr[3], v[3], dn[3];
+            else uy = (ly > 0) ? Ul[cur][ly - 1][lx] : sb_sweep_at<DIM, I>(a, x, y - 1, z);
+            T gr[3], v[3], dn[3], bv[3];
             gr[0] = ux - u2;
             gr[1] = uy - u2;
             gr[2] = 0;
@@ -129,7 +141,10 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
                 gr[2] = uz - u2;
             }
             #pragma unroll
-            for (int d = 0; d < DIM; ++d) v[d] = gr[d] + a.B[d][i];
+            for (int d = 0; d < DIM; ++d) {
+                bv[d] = ldx<T>(a.B[d], i);
+                v[d] = gr[d] + bv[d];
+            }
             if (ANISO) {
                 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
@@ -148,12 +163,12 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
             }
             #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                a.Dout[d][i] = dn[d];
-                a.Bout[d][i] = a.B[d][i] + (gr[d] - dn[d]);
+                stx<T>(a.Dout[d], i, dn[d]);
+                stx<T>(a.Bout[d], i, bv[d] + (gr[d] - dn[d]));
             }
-            a.Uout[i] = u2;
+            stx<T>(a.Uout, i, u2);
             if (a.check) {
-                const T u1 = a.Uin[i];
+                const T u1 = ldx<T>(a.Uin, i);
                 const T df = u2 - u1;
                 sdd += (double)df * (double)df;
                 suu += (double)u2 * (double)u2;
@@ -185,9 +200,12 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
         CILREG_CK(hipStreamSynchronize(s));
         return 0;
     }
-    int ty = 4, zc = opts.zchunk;   // the SB kernels are built for 64 x 4 tiles
-    auto_tiling(nx, ny, nz, &ty, &zc);
-    const Geom g = make_geom(nx, ny, nz, ty, zc);
+    const bool small = !opts.wide_index && fits_u32(n, sizeof(T));
+    // the SB kernels are built for 64 x 4 tiles
+    const Geom g = dim == 2 ? make_geom(nx, ny, nz, 4, 1)
+                            : plan_geom(small ? (const void*)sb_update_kernel<3, 4, T, false, uint32_t>
+                                              : (const void*)sb_update_kernel<3, 4, T, false, uint64_t>,
+                                        TX * 4, nx, ny, nz, 4, opts.zchunk);
     Arena ar;
     size_t oU0, oU1, oD[2][3], oB[2][3], oCtrl, oPart;
     oU0 = ar.reserve(n * sizeof(T));
@@ -230,23 +248,38 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
         a.partials = partials;
         a.ctrl = ctrl;
         a.g = g;
-        dim3 grid(g.nblocks);
+        dim3 grid(g.nblocks), block(TX, 4);
         // sweep 1: U0 -> U1
         a.Uin = (const T*)ar.at(oU0);
         a.Uout = (T*)ar.at(oU1);
         a.check = 0;
-        if (dim == 3) hipLaunchKernelGGL((sb_sweep_kernel<3, 4, T>), grid, dim3(TX, 4), 0, s, a);
-        else hipLaunchKernelGGL((sb_sweep_kernel<2, 4, T>), grid, dim3(TX, 4), 0, s, a);
+        if (small) {
+            if (dim == 3) hipLaunchKernelGGL((sb_sweep_kernel<3, 4, T, uint32_t>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((sb_sweep_kernel<2, 4, T, uint32_t>), grid, block, 0, s, a);
+        } else {
+            if (dim == 3) hipLaunchKernelGGL((sb_sweep_kernel<3, 4, T, uint64_t>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((sb_sweep_kernel<2, 4, T, uint64_t>), grid, block, 0, s, a);
+        }
         // sweep 2 + shrink + Bregman: U1 -> U2 (into U0's buffer)
         a.Uin = (const T*)ar.at(oU1);
         a.Uout = (T*)ar.at(oU0);
         a.check = check;
-        if (dim == 3) {
-            if (method == 1) hipLaunchKernelGGL((sb_update_kernel<3, 4, T, true>), grid, dim3(TX, 4), 0, s, a);
-            else hipLaunchKernelGGL((sb_update_kernel<3, 4, T, false>), grid, dim3(TX, 4), 0, s, a);
+        if (small) {
+            if (dim == 3) {
+                if (method == 1) hipLaunchKernelGGL((sb_update_kernel<3, 4, T, true, uint32_t>), grid, block, 0, s, a);
+                else hipLaunchKernelGGL((sb_update_kernel<3, 4, T, false, uint32_t>), grid, block, 0, s, a);
+            } else {
+                if (method == 1) hipLaunchKernelGGL((sb_update_kernel<2, 4, T, true, uint32_t>), grid, block, 0, s, a);
+                else hipLaunchKernelGGL((sb_update_kernel<2, 4, T, false, uint32_t>), grid, block, 0, s, a);
+            }
         } else {
-            if (method == 1) hipLaunchKernelGGL((sb_update_kernel<2, 4, T, true>), grid, dim3(TX, 4), 0, s, a);
-            else hipLaunchKernelGGL((sb_update_kernel<2, 4, T, false>), grid, dim3(TX, 4), 0, s, a);
+            if (dim == 3) {
+                if (method == 1) hipLaunchKernelGGL((sb_update_kernel<3, 4, T, true, uint64_t>), grid, block, 0, s, a);
+                else hipLaunchKernelGGL((sb_update_kernel<3, 4, T, false, uint64_t>), grid, block, 0, s, a);
+            } else {
+                if (method == 1) hipLaunchKernelGGL((sb_update_kernel<2, 4, T, true, uint64_t>), grid, block, 0, s, a);
+                else hipLaunchKernelGGL((sb_update_kernel<2, 4, T, false, uint64_t>), grid, block, 0, s, a);
+            }
         }
     };
     int done = 0;

@@ -276,8 +276,8 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         return 0;
     }
     const int nq = dim == 3 ? 6 : 3;
-    const bool small = fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
-    int ty = opts.tile_y == 8 && dim == 3 ? 8 : 4;
+    const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);

@@ -38,7 +38,8 @@ typedef struct cilreg_opts {
     int tile_y;          /* tile height in y (4 or 8); 0 = auto */
     int zchunk;          /* z planes marched per workgroup; 0 = auto */
     int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
-    int reserved[3];
+    int wide_index;      /* 1: force 64-bit element indexing (chosen automatically for fields >= 2 GiB) */
+    int reserved[2];
 } cilreg_opts;
 
 /* Last error recorded on this host thread ("" if none). */

@@ -102,6 +102,19 @@ def test_tiling_does_not_change_results(tile_y, zchunk):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", [c for c in CASES if c["input"] in ("p3d", "p2d") and c["params"]["tol"] == 0],
+                         ids=lambda c: case_id(c))
+def test_wide_index_kernels_match(case):
+    """The 64-bit-index instantiations (used for fields >= 2 GiB) give the same bits."""
+    import torch
+    from ccpi.filters import device as D
+    g = golden()
+    x = torch.from_numpy(g.input(case)).cuda()
+    y, info = D.run(case["method"], x, case["params"], wide_index=True)
+    _assert_parity(case, y.cpu().numpy(), info)
+
+
+@pytest.mark.gpu
 def test_zero_iterations():
     from ccpi.filters import regularisers as R
     x = golden().arrays["in_p3d"]
