@@ -18,7 +18,10 @@ Also reported:
 * secondary: TGV 3D 192x512x512 (500 iterations, FP32), PD-TV 3D 256^3
   (400 iterations, L = 12) and ROF-TV 2D 512^2 (2000 iterations) solves.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+       python bench.py --workload {tgv3d,pd3d,copy} --steps K   (single-workload runs for
+       profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
+       known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)
 """
 import argparse
 import json
@@ -153,6 +156,38 @@ def secondary(D, torch):
     return out
 
 
+def single_workload(name, steps, D, torch):
+    """One workload on its own (for profiler passes); prints a JSON line."""
+    from ccpi.supp import phantoms
+    if name == "copy":
+        n = 64 * 1024 * 1024   # 256 MiB of float32
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / max(steps, 1)
+        return {"workload": "copy 256 MiB f32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                "GBps": 8 * n / dt / 1e9}
+    if name == "tgv3d":
+        x = torch.from_numpy(phantoms.noisy(phantoms.ramp_ellipsoids_3d((192, 512, 512), 30, seed=3), 0.03,
+                                            seed=103)).cuda()
+        out = {}
+        for prec in ("fp32", "bf16", "fp16"):
+            D.tgv(x, 0.02, 1.0, 2.0, steps, 12.0, 0.0, precision=prec, profile=True)
+            m, _, c = D.last_profile()
+            out[prec] = {"kernel_ms_mean": m, "launches": c}
+        return {"workload": "TGV 3D 192x512x512", **out}
+    if name == "pd3d":
+        x = torch.from_numpy(fgp_workload()).cuda()
+        D.pd(x, 0.02, steps, 0.0, 12.0, 0, 1, profile=True)
+        m, _, c = D.last_profile()
+        return {"workload": "PD-TV 3D 256^3", "kernel_ms_mean": m, "launches": c}
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +196,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=16)
+    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -173,6 +209,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
+    if args.workload != "fgp3d":
+        print(json.dumps(single_workload(args.workload, args.steps, D, torch), default=float), flush=True)
+        return
 
     log("bench: FGP-TV 3D 256^3 phantom")
     x_np = fgp_workload()
