@@ -96,6 +96,20 @@ __device__ inline void stx(S* p, I e, T v) {
     Store<S>::st(IdxT<I>::at(p, e), 0, v);
 }
 
+// Non-temporal store of a field the current kernel never reads back: the
+// line is marked for early eviction, so the output stream does not push the
+// inputs' halo and z-neighbour lines out of L2 before they are reused.
+template <typename T, typename I, typename S>
+__device__ inline void stnt(S* p, I e, T v) {
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        bf16_t tmp;
+        Store<bf16_t>::st(&tmp, 0, (float)v);
+        __builtin_nontemporal_store(tmp.bits, &IdxT<I>::at(p, e)->bits);
+    } else {
+        __builtin_nontemporal_store((S)v, IdxT<I>::at(p, e));
+    }
+}
+
 // Largest element count addressed with 32-bit byte offsets for element size es.
 inline bool fits_u32(size_t n_elems, size_t es) { return n_elems * es < ((size_t)1 << 31); }
 

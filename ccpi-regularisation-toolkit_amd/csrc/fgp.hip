@@ -150,10 +150,10 @@ __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
                     p3 = p3 / a3;
                 }
             }
-            stx<T>(a.Pn[0], i, p1);
-            stx<T>(a.Pn[1], i, p2);
-            if (DIM == 3) stx<T>(a.Pn[2], i, p3);
-            if (a.write_d) stx<T>(a.Dout, i, d);
+            stnt<T>(a.Pn[0], i, p1);
+            stnt<T>(a.Pn[1], i, p2);
+            if (DIM == 3) stnt<T>(a.Pn[2], i, p3);
+            if (a.write_d) stnt<T>(a.Dout, i, d);
             if (a.check) {
                 const T df = d - ldx<T>(a.Dprev, i);
                 sdd += (double)df * (double)df;

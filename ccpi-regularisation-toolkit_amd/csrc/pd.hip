@@ -138,12 +138,12 @@ __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
                 const T df = un - u0;
                 sdd += (double)df * (double)df;
                 suu += (double)un * (double)un;
-                stx<T>(a.Ubrk, i, un);
+                stnt<T>(a.Ubrk, i, un);
             }
-            stx<T>(a.Uout, i, un + a.theta * (un - u0));
-            stx<T>(a.Pout[0], i, q1);
-            stx<T>(a.Pout[1], i, q2);
-            if (DIM == 3) stx<T>(a.Pout[2], i, q3);
+            stnt<T>(a.Uout, i, un + a.theta * (un - u0));
+            stnt<T>(a.Pout[0], i, q1);
+            stnt<T>(a.Pout[1], i, q2);
+            if (DIM == 3) stnt<T>(a.Pout[2], i, q3);
         }
         __syncthreads();
         p3below = q3;

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(TX * TY) void rof_iter_kernel(RofArgs<T> a) {
                 sdd += (double)df * (double)df;
                 suu += (double)un * (double)un;
             }
-            stx<T>(a.Uout, i, un);
+            stnt<T>(a.Uout, i, un);
         }
         __syncthreads();
         d3m = d3;

@@ -163,10 +163,10 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
             }
             #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                stx<T>(a.Dout[d], i, dn[d]);
-                stx<T>(a.Bout[d], i, bv[d] + (gr[d] - dn[d]));
+                stnt<T>(a.Dout[d], i, dn[d]);
+                stnt<T>(a.Bout[d], i, bv[d] + (gr[d] - dn[d]));
             }
-            stx<T>(a.Uout, i, u2);
+            stnt<T>(a.Uout, i, u2);
             if (a.check) {
                 const T u1 = ldx<T>(a.Uin, i);
                 const T df = u2 - u1;

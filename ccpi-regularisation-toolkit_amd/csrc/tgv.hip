@@ -47,23 +47,31 @@ struct TgvArgs {
 // and the +y neighbour (along y).
 enum { L_P1, L_P2, L_Q11, L_Q22, L_Q12, L_Q13, L_Q23, L_N };
 
-// Updated and projected duals at (px, py, z).  Every load is unconditional:
-// neighbour indices are clamped at the last index and the boundary rule is
-// applied with selects, so the ~25 loads issue back to back.
+// Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
+// the point) are passed in when the caller carries them from the previous
+// plane (carried = true) and loaded otherwise; uz / vz return u_bar and v_bar
+// one plane up, which are the next plane's own values.  Every load is
+// unconditional: neighbour indices are clamped at the last index and the
+// boundary rule is applied with selects, so the loads issue back to back.
 template <int DIM, typename I, typename T, typename S>
-__device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, T p[3], T q[6]) {
+__device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, bool carried, T& u, T v[3],
+                                   T& uz, T vz[3], T p[3], T q[6]) {
     const Geom& g = a.g;
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const I i = (I)z * sxy + (I)py * nx + (I)px;
     const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
     const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
     const T sigma = a.sigma;
-    const T u = ldx<T>(a.U, i), ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
-    const T uz = DIM == 3 ? ldx<T>(a.U, iz) : (T)0;
-    T v[3], vx[3], vy[3], vz[3], po[3], qo[6];
+    if (!carried) {
+        u = ldx<T>(a.U, i);
+        #pragma unroll
+        for (int c = 0; c < DIM; ++c) v[c] = ldx<T>(a.V[c], i);
+    }
+    const T ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
+    uz = DIM == 3 ? ldx<T>(a.U, iz) : (T)0;
+    T vx[3], vy[3], po[3], qo[6];
     #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-        v[c] = ldx<T>(a.V[c], i);
         vx[c] = ldx<T>(a.V[c], ix);
         vy[c] = ldx<T>(a.V[c], iy);
         if (DIM == 3) vz[c] = ldx<T>(a.V[c], iz);
@@ -167,15 +175,19 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
 
     // channels of plane z-1 at the own point: P3, Q33, Q13, Q23
     T zp3 = 0, zq33 = 0, zq13 = 0, zq23 = 0;
+    // u_bar, v_bar of the own and the halo point carried from the plane below
+    T cu = 0, cv[3] = {0, 0, 0}, hu = 0, hv[3] = {0, 0, 0};
+    bool own_carried = false;
     if (DIM == 3 && tc.z0 > 0) {
-        T p[3], q[6];
-        tgv_dual_at<DIM, I>(a, ox, oy, tc.z0 - 1, p, q);
+        T p[3], q[6], u0, v0[3];
+        tgv_dual_at<DIM, I>(a, ox, oy, tc.z0 - 1, false, u0, v0, cu, cv, p, q);
         zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
+        own_carried = true;
     }
     double sdd = 0.0, suu = 0.0;
     for (int z = tc.z0; z < tc.z1; ++z) {
-        T p[3], q[6];
-        tgv_dual_at<DIM, I>(a, ox, oy, z, p, q);
+        T p[3], q[6], nu = 0, nv[3] = {0, 0, 0};
+        tgv_dual_at<DIM, I>(a, ox, oy, z, own_carried || z > tc.z0, cu, cv, nu, nv, p, q);
         {
             const int r = ly + 1, c = lx + 1;
             L[L_P1][r][c] = p[0];
@@ -190,9 +202,10 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
                 L[L_Q12][r][c] = q[2];
             }
         }
+        T nhu = 0, nhv[3] = {0, 0, 0};
         if (hlane) {
             T hp[3], hq[6];
-            tgv_dual_at<DIM, I>(a, hcx, hcy, z, hp, hq);
+            tgv_dual_at<DIM, I>(a, hcx, hcy, z, z > tc.z0, hu, hv, nhu, nhv, hp, hq);
             L[L_P1][hly][hlx] = hp[0];
             L[L_P2][hly][hlx] = hp[1];
             L[L_Q11][hly][hlx] = hq[0];
@@ -211,7 +224,7 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
             // x-predecessor at [ly+1][lx], y-predecessor at [ly][lx+1]
             T dv = bdiff(p[0], L[L_P1][ly + 1][lx], x, g.nx) + bdiff(p[1], L[L_P2][ly][lx + 1], y, g.ny);
             if (DIM == 3) dv = dv + bdiff(p[2], zp3, z, g.nz);
-            const T ub = ldx<T>(a.U, i);
+            const T ub = cu;
             const T un = (a.lambda * (ub + a.tau * dv) + a.tau * ldx<T>(a.f, i)) / (a.lambda + a.tau);
             const T ubn = (T)2 * un - ub;
             if (a.check) {
@@ -219,7 +232,7 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
                 sdd += (double)df * (double)df;
                 suu += (double)ubn * (double)ubn;
             }
-            stx<T>(a.Uout, i, ubn);
+            stnt<T>(a.Uout, i, ubn);
             T dq[3];
             if (DIM == 3) {
                 dq[0] = bdiff(q[0], L[L_Q11][ly + 1][lx], x, g.nx) + bdiff(q[3], L[L_Q12][ly][lx + 1], y, g.ny) +
@@ -235,17 +248,20 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
             }
             #pragma unroll
             for (int c = 0; c < DIM; ++c) {
-                const T vb = ldx<T>(a.V[c], i);
+                const T vb = cv[c];
                 const T vn = vb + a.tau * (p[c] + dq[c]);
-                stx<T>(a.Vout[c], i, (T)2 * vn - vb);
-                stx<T>(a.Pout[c], i, p[c]);
+                stnt<T>(a.Vout[c], i, (T)2 * vn - vb);
+                stnt<T>(a.Pout[c], i, p[c]);
             }
             const int nq = DIM == 3 ? 6 : 3;
             #pragma unroll
-            for (int c = 0; c < nq; ++c) stx<T>(a.Qout[c], i, q[c]);
+            for (int c = 0; c < nq; ++c) stnt<T>(a.Qout[c], i, q[c]);
         }
         __syncthreads();
         zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
+        cu = nu; hu = nhu;
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) { cv[c] = nv[c]; hv[c] = nhv[c]; }
     }
     if (a.check) {
         block_sum2(sdd, suu, red);
