@@ -1,0 +1,120 @@
+"""Checks on the CPU oracle that the mathematics fixes, independent of the
+reference fixtures.
+
+* Scale equivariance, bit-exact for a factor of 2 (every scaling by 2 is exact
+  in binary floating point and each solver's step sizes scale accordingly):
+  FGP(2f, 2 lambda) = 2 FGP(f, lambda), PD likewise, SB(2f, 2 mu) = 2 SB(f, mu),
+  TGV(2f; lambda, 2 alpha1, 2 alpha0) = 2 TGV(f; lambda, alpha1, alpha0).
+  A dropped or misplaced step-size factor breaks these.
+* Constant images are fixed points: exactly for ROF and FGP, to a few ulps for
+  PD, SB and TGV (whose prox steps divide c (1 + t) by (1 + t)).
+* TGV in 3D with a singleton y axis equals TGV in 2D on the (z, x) plane, bit
+  for bit: every y-derivative is zero, so P2, V2, Q22, Q12, Q23 stay zero and
+  the remaining terms are summed in the same order.  A transposed operand or a
+  wrong component in the symmetric gradient breaks this.
+* Zero iterations return the input (the FGP output buffer is left alone).
+* The FP64 build agrees with the FP32 build to FP32 accuracy.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import bits_equal
+
+RNG = np.random.default_rng(7)
+F2 = RNG.random((23, 31)).astype(np.float32)
+F3 = RNG.random((7, 9, 13)).astype(np.float32)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+@pytest.mark.parametrize("method,nonneg", [(0, 0), (1, 1)])
+def test_fgp_scale_equivariance(f, method, nonneg):
+    lib = oracle.cpu_ref()
+    a, _ = lib.fgp(f, 0.03, 40, 0.0, method, nonneg)
+    b, _ = lib.fgp(2 * f, 0.06, 40, 0.0, method, nonneg)
+    assert bits_equal(b, 2 * a)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+@pytest.mark.parametrize("method,nonneg", [(0, 1), (1, 0)])
+def test_pd_scale_equivariance(f, method, nonneg):
+    lib = oracle.cpu_ref()
+    a, _ = lib.pd(f, 0.03, 40, 0.0, 8.0, method, nonneg)
+    b, _ = lib.pd(2 * f, 0.06, 40, 0.0, 8.0, method, nonneg)
+    assert bits_equal(b, 2 * a)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+@pytest.mark.parametrize("method", [0, 1])
+def test_sb_scale_equivariance(f, method):
+    lib = oracle.cpu_ref()
+    a, _ = lib.sb(f, 0.05, 20, 0.0, method)
+    b, _ = lib.sb(2 * f, 0.1, 20, 0.0, method)
+    assert bits_equal(b, 2 * a)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+def test_tgv_scale_equivariance(f):
+    lib = oracle.cpu_ref()
+    a, _ = lib.tgv(f, 0.05, 1.0, 2.0, 40, 12.0, 0.0)
+    b, _ = lib.tgv(2 * f, 0.05, 2.0, 4.0, 40, 12.0, 0.0)
+    assert bits_equal(b, 2 * a)
+
+
+@pytest.mark.parametrize("shape", [(17, 19), (5, 6, 7)], ids=["2d", "3d"])
+def test_constant_image_is_fixed(shape):
+    lib = oracle.cpu_ref()
+    c = np.full(shape, 0.37, dtype=np.float32)
+    y, _ = lib.rof(c, 0.02, 30, 0.001, 0.0)
+    assert np.array_equal(y, c)
+    y, _ = lib.fgp(c, 0.02, 30, 0.0, 0, 1)
+    assert np.array_equal(y, c)
+    for y in (lib.pd(c, 0.02, 30, 0.0, 12.0, 0, 0)[0], lib.sb(c, 0.02, 30, 0.0, 0)[0],
+              lib.tgv(c, 0.02, 1.0, 2.0, 30, 12.0, 0.0)[0]):
+        assert np.max(np.abs(y - c)) <= 8 * np.spacing(np.float32(0.37))
+
+
+def test_tgv_singleton_axis_reduces_to_2d():
+    lib = oracle.cpu_ref()
+    f2 = RNG.random((11, 14)).astype(np.float32)           # (Y', X) plane
+    f3 = np.ascontiguousarray(f2[:, None, :])                # (Z, 1, X) volume
+    a, ia = lib.tgv(f2, 0.05, 1.0, 2.0, 60, 12.0, 0.0)
+    b, ib = lib.tgv(f3, 0.05, 1.0, 2.0, 60, 12.0, 0.0)
+    assert bits_equal(b[:, 0, :], a)
+
+
+def test_zero_iterations_identity():
+    lib = oracle.cpu_ref()
+    for y in (lib.rof(F3, 0.02, 0, 0.001, 0.0)[0], lib.pd(F3, 0.02, 0, 0.0, 8.0, 0, 0)[0],
+              lib.sb(F3, 0.02, 0, 0.0, 0)[0], lib.tgv(F3, 0.02, 1.0, 2.0, 0, 12.0, 0.0)[0]):
+        assert bits_equal(y, F3)
+    out = np.full_like(F3, 5.0)
+    y, info = lib.fgp(F3, 0.02, 0, 1e-3, 0, 0, out=out)
+    assert np.all(y == 5.0) and info[0] == 0
+
+
+@pytest.mark.parametrize("method", ["rof", "fgp", "pd", "sb", "tgv"])
+def test_fp64_build_agrees_with_fp32(method):
+    params = {
+        "rof": {"lambda": 0.02, "iters": 50, "tau": 0.001, "tol": 0.0},
+        "fgp": {"lambda": 0.02, "iters": 50, "tol": 0.0, "method": 0, "nonneg": 1},
+        "pd": {"lambda": 0.02, "iters": 50, "tol": 0.0, "L": 12.0, "method": 0, "nonneg": 1},
+        "sb": {"mu": 0.02, "iters": 20, "tol": 0.0, "method": 0},
+        "tgv": {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 50, "L2": 12.0, "tol": 0.0},
+    }[method]
+    a, _ = oracle.cpu_ref("f32").run(method, F3, params)
+    b, _ = oracle.cpu_ref("f64").run(method, F3.astype(np.float64), params)
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert err < 1e-5, err
+
+
+def test_tv_energy_matches_direct_sum():
+    u = RNG.random((4, 5, 6)).astype(np.float32)
+    u0 = RNG.random((4, 5, 6)).astype(np.float32)
+    e = oracle.tv_energy_f64(u, u0, 0.1)
+    ud = u.astype(np.float64)
+    gx = np.diff(ud, axis=2, append=ud[:, :, -1:])
+    gy = np.diff(ud, axis=1, append=ud[:, -1:, :])
+    gz = np.diff(ud, axis=0, append=ud[-1:, :, :])
+    direct = np.sum(2 * 0.1 * np.sqrt(gx ** 2 + gy ** 2 + gz ** 2)) + np.sum((ud - u0) ** 2)
+    assert abs(e - direct) <= 1e-9 * abs(direct)
