@@ -98,32 +98,55 @@ def secondary(D, torch):
     """TGV 3D, PD 3D and ROF 2D solves (one timed run each after one warm-up)."""
     from ccpi.supp import phantoms
     out = {}
-    # TGV 3D precision-study volume, FP32 duals
+    # TGV 3D precision study: FP32 / BF16 / FP16 storage of P, Q, v_bar against an FP64 solve
     log("bench: TGV 3D 192x512x512 phantom")
     clean = phantoms.ramp_ellipsoids_3d((192, 512, 512), 30, seed=3)
     x = torch.from_numpy(phantoms.noisy(clean, 0.03, seed=103)).cuda()
     del clean
     n = x.numel()
-    D.tgv(x, 0.02, 1.0, 2.0, 20, 12.0, 0.0)
+    tgv_args = (0.02, 1.0, 2.0)   # lambda (reference tests' value), alpha1, alpha0; L2 = 12
+
+    def objective(u):
+        # E = sum 2 lambda |grad+ u| + sum (u - f)^2 in FP64, replicate boundary (utils.c:89-117)
+        ud = u.double()
+        gx = torch.diff(ud, dim=2, append=ud[:, :, -1:])
+        gy = torch.diff(ud, dim=1, append=ud[:, -1:, :])
+        gz = torch.diff(ud, dim=0, append=ud[-1:, :, :])
+        e = (2 * tgv_args[0] * torch.sqrt(gx * gx + gy * gy + gz * gz)).sum() + ((ud - x.double()) ** 2).sum()
+        return float(e)
+
+    D.tgv(x, *tgv_args, 20, 12.0, 0.0)
+    results = {}
+    for prec in ("fp32", "bf16", "fp16"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y, info = D.tgv(x, *tgv_args, 500, 12.0, 0.0, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        D.tgv(x, *tgv_args, 50, 12.0, 0.0, precision=prec, profile=True)
+        m, _, _ = D.last_profile()
+        b = 108 if prec == "fp32" else 60
+        results[prec] = {"y": y, "voxel_iters_per_s": n * info[0] / dt, "seconds": round(dt, 4),
+                         "iterations": int(info[0]), "kernel_ms_mean": m, "alg_bytes_per_voxel_iter": b,
+                         "kernel_alg_GBps": b * n / (m * 1e-3) / 1e9 if m else None}
+    log("bench: TGV FP64 reference solve")
+    x64 = x.double()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, info = D.tgv(x, 0.02, 1.0, 2.0, 500, 12.0, 0.0)
+    ref, _ = D.tgv_f64(x64, *tgv_args, 500, 12.0, 0.0)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    D.tgv(x, 0.02, 1.0, 2.0, 50, 12.0, 0.0, profile=True)
-    mean_ms, min_ms, cnt = D.last_profile()
-    out["tgv_3d_192x512x512_fp32"] = {
-        "voxel_iters_per_s": n * info[0] / dt, "iterations": int(info[0]), "seconds": round(dt, 4),
-        "kernel_ms_mean": mean_ms, "alg_bytes_per_voxel_iter": 108,
-        "kernel_alg_GBps": 108 * n / (mean_ms * 1e-3) / 1e9 if mean_ms else None,
-        "kernel_frac_of_8TBps": 108 * n / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if mean_ms else None}
-    for prec in ("bf16", "fp16"):
-        D.tgv(x, 0.02, 1.0, 2.0, 50, 12.0, 0.0, precision=prec, profile=True)
-        m, _, _ = D.last_profile()
-        out[f"tgv_3d_192x512x512_{prec}_duals"] = {
-            "kernel_ms_mean": m, "alg_bytes_per_voxel_iter": 60,
-            "voxel_iters_per_s_kernel": n / (m * 1e-3) if m else None}
-    del x
+    t64 = time.perf_counter() - t0
+    del x64
+    ref_norm = float(torch.linalg.vector_norm(ref))
+    for prec, r in results.items():
+        y = r.pop("y")
+        r["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(y.double() - ref)) / ref_norm
+        r["objective_fp64"] = objective(y)
+        out[f"tgv_3d_192x512x512_{prec}"] = r
+    out["tgv_3d_192x512x512_fp64_reference"] = {"seconds": round(t64, 4), "objective_fp64": objective(ref)}
+    out["tgv_3d_192x512x512_fp32"]["kernel_frac_of_8TBps"] = (
+        out["tgv_3d_192x512x512_fp32"]["kernel_alg_GBps"] / HBM_PEAK_GBS)
+    del ref, x
     # PD 3D
     log("bench: PD-TV 3D 256^3")
     x = torch.from_numpy(fgp_workload()).cuda()

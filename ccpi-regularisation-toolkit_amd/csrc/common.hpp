@@ -51,13 +51,9 @@ template <> struct Store<bf16_t> {
     __device__ static inline float ld(const bf16_t* p, size_t i) {
         return __uint_as_float(((uint32_t)p[i].bits) << 16);
     }
-    __device__ static inline void st(bf16_t* p, size_t i, float v) {
-        uint32_t u = __float_as_uint(v);
-        uint16_t r;
-        if ((u & 0x7fffffffu) > 0x7f800000u) r = (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
-        else r = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);               // round to nearest even
-        p[i].bits = r;
-    }
+    // round to nearest even (gfx950 converts in hardware, v_cvt_pk_bf16_f32)
+    __device__ static inline uint16_t bits_of(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+    __device__ static inline void st(bf16_t* p, size_t i, float v) { p[i].bits = bits_of(v); }
 };
 
 template <typename T, typename S>
@@ -102,9 +98,7 @@ __device__ inline void stx(S* p, I e, T v) {
 template <typename T, typename I, typename S>
 __device__ inline void stnt(S* p, I e, T v) {
     if constexpr (std::is_same<S, bf16_t>::value) {
-        bf16_t tmp;
-        Store<bf16_t>::st(&tmp, 0, (float)v);
-        __builtin_nontemporal_store(tmp.bits, &IdxT<I>::at(p, e)->bits);
+        __builtin_nontemporal_store(Store<bf16_t>::bits_of((float)v), &IdxT<I>::at(p, e)->bits);
     } else {
         __builtin_nontemporal_store((S)v, IdxT<I>::at(p, e));
     }

@@ -1,0 +1,77 @@
+"""FP64 solves and reduced-precision dual storage on the GPU.
+
+* The FP64 instantiations (cilreg_fgp_dev_f64, cilreg_tgv_dev_f64) are the
+  oracle of the precision study; they must equal the FP64 build of the CPU
+  restatement bit for bit (same operation order, IEEE double division and
+  sqrt, no contraction).
+* BF16 / FP16 storage of the dual and auxiliary fields (FGP: P, P_prev; PD: P;
+  TGV: P, Q, v_bar) with FP32 arithmetic: relative L2 error of the output
+  against the FP64 solve, with the bounds stated below (FP32 storage 1e-5,
+  FP16 1e-3, BF16 3e-3 at these sizes and iteration counts).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden, rel_l2
+
+BOUND = {"fp32": 1e-5, "fp16": 1e-3, "bf16": 3e-3}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("inp", ["p2d", "p3d"])
+def test_fp64_gpu_matches_fp64_oracle(inp):
+    import torch
+    from ccpi.filters import device as D
+    x = golden().arrays["in_" + inp].astype(np.float64)
+    xt = torch.from_numpy(x).cuda()
+    a, ia = D.tgv_f64(xt, 0.05, 1.0, 2.0, 60, 12.0, 0.0)
+    b, ib = oracle.cpu_ref("f64").tgv(x, 0.05, 1.0, 2.0, 60, 12.0, 0.0)
+    assert np.array_equal(a.cpu().numpy(), b)
+    a, ia = D.fgp_f64(xt, 0.03, 60, 0.0, 0, 1)
+    b, ib = oracle.cpu_ref("f64").fgp(x, 0.03, 60, 0.0, 0, 1)
+    assert np.array_equal(a.cpu().numpy(), b)
+
+
+def _phantom():
+    import sys
+    from ccpi.supp import phantoms
+    clean = phantoms.ramp_ellipsoids_3d((40, 96, 112), 12, seed=31)
+    return phantoms.noisy(clean, 0.03, seed=32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_tgv_reduced_precision_duals(prec):
+    import torch
+    from ccpi.filters import device as D
+    x = _phantom()
+    ref, _ = D.tgv_f64(torch.from_numpy(x.astype(np.float64)).cuda(), 0.02, 1.0, 2.0, 300, 12.0, 0.0)
+    y, info = D.tgv(torch.from_numpy(x).cuda(), 0.02, 1.0, 2.0, 300, 12.0, 0.0, precision=prec)
+    err = rel_l2(y.cpu().numpy(), ref.cpu().numpy())
+    print(f"TGV 40x96x112, 300 iterations, {prec} duals: rel L2 vs FP64 = {err:.3e}")
+    assert err <= BOUND[prec], err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_fgp_reduced_precision_duals(prec):
+    import torch
+    from ccpi.filters import device as D
+    x = _phantom()
+    ref, _ = D.fgp_f64(torch.from_numpy(x.astype(np.float64)).cuda(), 0.02, 200, 0.0, 0, 1)
+    y, info = D.fgp(torch.from_numpy(x).cuda(), 0.02, 200, 0.0, 0, 1, precision=prec)
+    err = rel_l2(y.cpu().numpy(), ref.cpu().numpy())
+    print(f"FGP 40x96x112, 200 iterations, {prec} duals: rel L2 vs FP64 = {err:.3e}")
+    assert err <= BOUND[prec], err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_reduced_precision_is_deterministic(prec):
+    import torch
+    from ccpi.filters import device as D
+    x = torch.from_numpy(_phantom()).cuda()
+    a, _ = D.tgv(x, 0.02, 1.0, 2.0, 30, 12.0, 0.0, precision=prec)
+    b, _ = D.tgv(x, 0.02, 1.0, 2.0, 30, 12.0, 0.0, precision=prec, tile_y=4, zchunk=3)
+    assert torch.equal(a, b)
