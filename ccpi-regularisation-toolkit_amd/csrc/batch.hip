@@ -1,0 +1,965 @@
+// batch.hip -- grouped multi-volume execution: the batch runner, the
+// continuous-batching server and the device phantom generator.
+//
+// A *job* is one volume with its method, parameters and device state.  Jobs of
+// the same method, dimensionality, projection type and index width form a
+// *group*; one launch of the group's kernel advances every job of the group
+// by one iteration: the grid is the concatenation of the jobs' tile grids,
+// and each workgroup finds its job by binary search over the jobs' first
+// block (after the XCD-aware remap of the whole grid, so each XCD works on a
+// contiguous run of jobs).  Each job carries its own start level, iteration
+// count, stop flag and per-block FP64 partials, so jobs at different
+// iteration counts share launches and a job's arithmetic -- and therefore its
+// result -- is the same whether it runs alone or in any mix.
+//
+// The per-tile bodies are the single-volume kernels' (`*_kernels.hpp`); only
+// the argument block is built per workgroup from the job descriptor and the
+// job's iteration index k = level - start_level.
+#include "solvers.hpp"
+#include "rof_kernels.hpp"
+#include "fgp_kernels.hpp"
+#include "pd_kernels.hpp"
+#include "tgv_kernels.hpp"
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace cilreg {
+
+// ---------------------------------------------------------------------------
+// device job descriptors
+// ---------------------------------------------------------------------------
+struct JobHdr {
+    Geom g;
+    int block_offset;   // first logical block of the job in its group's grid
+    int start_level;    // level at which the job runs its iteration 0
+    int iters;          // iteration count
+    int check_every;    // residual cadence, 0 = no checks
+    Ctrl* ctrl;
+    double* partials;
+};
+
+template <typename T> struct RofJob {
+    JobHdr h;
+    const T* f;
+    T* U[2];
+    T lambda, tau;
+};
+template <typename T, typename S> struct FgpJob {
+    JobHdr h;
+    const T* f;
+    S* P[3][3];
+    T* D[2];
+    const T* mult;      // FISTA multipliers by iteration index
+    T lambda, step;
+    int nonneg;
+};
+template <typename T, typename S> struct PdJob {
+    JobHdr h;
+    const T* f;
+    T* U[2];
+    S* P[2][3];
+    T* Ubrk;
+    T sigma, tau, lt;
+    int nonneg;
+};
+template <typename T, typename S> struct TgvJob {
+    JobHdr h;
+    const T* f;
+    T* U[2];
+    S* V[2][3];
+    S* P[2][3];
+    S* Q[2][6];
+    T lambda, tau, alpha1, alpha0;
+};
+
+struct FinJob {
+    Ctrl* ctrl;
+    const double* partials;
+    int nblocks, start_level, iters, check_every;
+    float epsil;
+};
+
+template <class J>
+__device__ inline int find_job(const J* jobs, int n, int L) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].h.block_offset <= L) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Common prologue of every grouped kernel: the job of this workgroup and its
+// iteration index, or -1 when the job does not run at this level.
+template <class J>
+__device__ inline int group_prologue(const J* jobs, int njobs, int total_blocks, int level, const J*& job,
+                                     int& local) {
+    const int L = xcd_logical_block((int)blockIdx.x, total_blocks);
+    const int j = find_job(jobs, njobs, L);
+    job = &jobs[j];
+    const int k = level - job->h.start_level;
+    if (k < 0 || k >= job->h.iters) return -1;
+    if (job->h.ctrl->stopped) return -1;
+    local = L - job->h.block_offset;
+    return k;
+}
+
+__device__ inline bool job_checks(const JobHdr& h, int k) { return h.check_every > 0 && k % h.check_every == 0; }
+
+template <int DIM, int TY, typename T, typename I>
+__global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* jobs, int njobs, int total, int level) {
+    const RofJob<T>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    RofArgs<T> a;
+    a.f = j->f;
+    a.U = j->U[k & 1];
+    a.Uout = j->U[(k + 1) & 1];
+    a.lambda = j->lambda;
+    a.tau = j->tau;
+    a.check = job_checks(j->h, k);
+    a.partials = j->h.partials;
+    a.ctrl = j->h.ctrl;
+    a.g = j->h.g;
+    rof_body<DIM, TY, T, I>(a, tile_of_logical(a.g, TY, local));
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total,
+                                                            int level) {
+    const FgpJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    FgpArgs<T, S> a;
+    const int cur = k % 3, prv = (k + 2) % 3, nxt = (k + 1) % 3;
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.P[c] = j->P[cur][c];
+        a.Pp[c] = j->P[prv][c];
+        a.Pn[c] = j->P[nxt][c];
+    }
+    const bool check = job_checks(j->h, k);
+    a.f = j->f;
+    a.Dout = j->D[k & 1];
+    a.Dprev = j->D[(k + 1) & 1];
+    a.lambda = j->lambda;
+    a.step = j->step;
+    a.mult = j->mult[k];
+    a.write_d = (k == j->h.iters - 1) || check || (j->h.check_every > 0 && (k + 1) % j->h.check_every == 0);
+    a.check = check;
+    a.nonneg = j->nonneg;
+    a.partials = j->h.partials;
+    a.ctrl = j->h.ctrl;
+    a.g = j->h.g;
+    fgp_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+    const PdJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    PdArgs<T, S> a;
+    const int cur = k & 1, nxt = cur ^ 1;
+    a.f = j->f;
+    a.U = j->U[cur];
+    a.Uout = j->U[nxt];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.P[c] = j->P[cur][c];
+        a.Pout[c] = j->P[nxt][c];
+    }
+    a.Ubrk = j->Ubrk;
+    a.sigma = j->sigma;
+    a.tau = j->tau;
+    a.lt = j->lt;
+    a.theta = 1;
+    a.nonneg = j->nonneg;
+    a.check = job_checks(j->h, k);
+    a.partials = j->h.partials;
+    a.ctrl = j->h.ctrl;
+    a.g = j->h.g;
+    pd_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
+}
+
+template <int DIM, int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total,
+                                                            int level) {
+    const TgvJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    TgvArgs<T, S> a;
+    const int cur = k & 1, nxt = cur ^ 1;
+    a.f = j->f;
+    a.U = j->U[cur];
+    a.Uout = j->U[nxt];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.V[c] = j->V[cur][c];
+        a.Vout[c] = j->V[nxt][c];
+        a.P[c] = j->P[cur][c];
+        a.Pout[c] = j->P[nxt][c];
+    }
+    #pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        a.Q[c] = j->Q[cur][c];
+        a.Qout[c] = j->Q[nxt][c];
+    }
+    a.lambda = j->lambda;
+    a.tau = j->tau;
+    a.sigma = j->tau;
+    a.alpha1 = j->alpha1;
+    a.alpha0 = j->alpha0;
+    a.check = job_checks(j->h, k);
+    a.partials = j->h.partials;
+    a.ctrl = j->h.ctrl;
+    a.g = j->h.g;
+    tgv_body<DIM, TY, T, S, I>(a, tile_of_logical(a.g, TY, local));
+}
+
+__global__ __launch_bounds__(256) void group_finalize_kernel(const FinJob* jobs, int level) {
+    const FinJob& j = jobs[blockIdx.x];
+    const int k = level - j.start_level;
+    if (k < 0 || k >= j.iters || j.check_every <= 0 || k % j.check_every != 0) return;
+    finalize_body(j.partials, j.nblocks, j.ctrl, j.epsil, k);
+}
+
+// ---------------------------------------------------------------------------
+// device phantom generator
+// ---------------------------------------------------------------------------
+struct Shape {
+    float c[3], r[3];
+    float value;
+    int rect;
+};
+
+__device__ __host__ inline uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__device__ __host__ inline double unit_of(uint64_t v) { return (double)(v >> 11) * (1.0 / 9007199254740992.0); }
+
+__global__ __launch_bounds__(256) void phantom_kernel(float* out, int nx, int ny, int nz, const Shape* shapes,
+                                                      int n_shapes, float sigma, uint64_t seed) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int x = (int)(i % nx);
+        const int y = (int)((i / nx) % ny);
+        const int z = (int)(i / ((size_t)nx * ny));
+        const float p[3] = {(x + 0.5f) / nx, (y + 0.5f) / ny, (z + 0.5f) / nz};
+        float v = 0.f;
+        for (int s = 0; s < n_shapes; ++s) {
+            const Shape sh = shapes[s];
+            bool in;
+            if (sh.rect) {
+                in = fabsf(p[0] - sh.c[0]) <= sh.r[0] && fabsf(p[1] - sh.c[1]) <= sh.r[1] &&
+                     (nz == 1 || fabsf(p[2] - sh.c[2]) <= sh.r[2]);
+            } else {
+                float q = 0.f;
+                for (int d = 0; d < (nz == 1 ? 2 : 3); ++d) {
+                    const float t = (p[d] - sh.c[d]) / sh.r[d];
+                    q += t * t;
+                }
+                in = q <= 1.f;
+            }
+            if (in && sh.value > v) v = sh.value;
+        }
+        // Box-Muller on a counter-based stream: deterministic for (seed, voxel)
+        uint64_t st = seed ^ (0xd1b54a32d192ed03ULL * (uint64_t)(i + 1));
+        const double u1 = fmax(unit_of(splitmix64(st)), 1e-300), u2 = unit_of(splitmix64(st));
+        const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        out[i] = v + sigma * (float)g;
+    }
+}
+
+static void make_shapes(int n_shapes, bool is2d, uint64_t seed, std::vector<Shape>& out) {
+    uint64_t st = seed * 0x2545f4914f6cdd1dULL + 1;
+    out.resize(n_shapes);
+    for (auto& sh : out) {
+        for (int d = 0; d < 3; ++d) sh.c[d] = (float)unit_of(splitmix64(st));
+        for (int d = 0; d < 3; ++d) sh.r[d] = 0.05f + 0.25f * (float)unit_of(splitmix64(st));
+        sh.value = (float)unit_of(splitmix64(st));
+        sh.rect = is2d ? (unit_of(splitmix64(st)) < 0.5) : 0;
+    }
+}
+
+static int launch_phantom(float* d_out, int nx, int ny, int nz, int n_shapes, float sigma, uint64_t seed,
+                          Shape* d_shapes, hipStream_t s) {
+    std::vector<Shape> shapes;
+    make_shapes(n_shapes, nz <= 1, seed, shapes);
+    if (n_shapes > 0)
+        CILREG_CK(hipMemcpyAsync(d_shapes, shapes.data(), n_shapes * sizeof(Shape), hipMemcpyHostToDevice, s));
+    const size_t n = (size_t)nx * ny * (nz < 1 ? 1 : nz);
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(phantom_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_out, nx, ny, nz < 1 ? 1 : nz,
+                       (const Shape*)d_shapes, n_shapes, sigma, (uint64_t)seed);
+    // the host vector is released on return: wait for the upload
+    CILREG_CK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the engine shared by the batch runner and the server
+// ---------------------------------------------------------------------------
+enum JobState { J_PENDING = 0, J_ACTIVE = 1, J_DONE = 2, J_FAILED = 3 };
+
+struct EJob {
+    cilreg_job pub;
+    long long ticket = 0;
+    int state = J_PENDING;
+    int dim = 3, nz = 1, ty = 8;
+    bool wide = false;
+    size_t n = 0, bytes = 0;
+    std::unique_ptr<Arena> ar;   // every device buffer of the job, one allocation
+    size_t off_fld[32] = {}, off_ctrl = 0, off_part = 0, off_f = 0, off_sh = 0;
+    int nfld = 0;
+    // device addresses (inside ar unless the caller provides the input)
+    const float* f = nullptr;
+    void* fld[32] = {};   // method state buffers, in the order plan_job reserves them
+    Ctrl* ctrl = nullptr;
+    double* partials = nullptr;
+    Shape* shapes = nullptr;
+    Geom g{};
+    int start_level = 0;
+    const float* result = nullptr;   // final buffer once done
+    float info[2] = {0, 0};
+};
+
+// group key: method, dimensionality, projection, index width
+struct GroupKey {
+    int method, dim, iso, wide;
+    bool operator<(const GroupKey& o) const {
+        if (method != o.method) return method < o.method;
+        if (dim != o.dim) return dim < o.dim;
+        if (iso != o.iso) return iso < o.iso;
+        return wide < o.wide;
+    }
+};
+
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual int submit(const cilreg_job& j, long long* ticket) = 0;
+    virtual int step(int levels, int* n_active, int* n_pending) = 0;
+    virtual int poll(long long ticket, float* info, float* d_dst) = 0;
+    virtual int run_batch(cilreg_job* jobs, int n) = 0;
+};
+
+template <typename S>
+class Engine : public EngineBase {
+public:
+    Engine(int dev, double budget, const Opts& o, hipStream_t s) : dev_(dev), budget_(budget), opts_(o), s_(s) {}
+    ~Engine() override {
+        for (auto& g : side_) (void)hipStreamDestroy(g);
+        for (auto& e : events_) (void)hipEventDestroy(e);
+        if (mult_) pool_free(mult_);
+        for (auto& kv : tables_) if (kv.second.dev) pool_free(kv.second.dev);
+        if (fin_dev_) pool_free(fin_dev_);
+    }
+
+    int submit(const cilreg_job& pj, long long* ticket) override {
+        if (pj.method < CILREG_ROF || pj.method > CILREG_TGV) { set_error("unknown method %d", pj.method); return -1; }
+        if (pj.dimX <= 0 || pj.dimY <= 0 || pj.dimZ < 0) { set_error("invalid dimensions"); return -1; }
+        if (!pj.d_input && pj.phantom_shapes <= 0) { set_error("job has neither d_input nor a phantom"); return -1; }
+        auto j = std::make_unique<EJob>();
+        j->pub = pj;
+        j->ticket = ++next_ticket_;
+        j->dim = pj.dimZ > 1 ? 3 : 2;
+        if (j->pub.method == CILREG_PD && j->pub.lipschitz <= 0.f) j->pub.lipschitz = 8.f;
+        if (j->pub.method == CILREG_TGV && j->pub.L2 <= 0.f) j->pub.L2 = 12.f;
+        j->nz = j->dim == 2 ? 1 : pj.dimZ;
+        j->n = (size_t)pj.dimX * pj.dimY * j->nz;
+        j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
+        j->ty = j->dim == 3 ? 8 : 4;
+        const int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
+        j->g = make_geom(pj.dimX, pj.dimY, j->nz, j->ty, zc);
+        plan_job(*j);
+        j->bytes = j->ar->bytes();
+        if (ticket) *ticket = j->ticket;
+        pending_.push_back(j.get());
+        all_[j->ticket] = std::move(j);
+        return 0;
+    }
+
+    int step(int levels, int* n_active, int* n_pending) override {
+        if (admit() != 0) return -1;
+        if (!active_.empty() && levels > 0) {
+            if (build_tables() != 0) return -1;
+            if (launch_levels(level_, level_ + levels, false) != 0) return -1;
+            level_ += levels;
+            CILREG_CK(hipStreamSynchronize(s_));
+            if (retire() != 0) return -1;
+        }
+        if (n_active) *n_active = (int)active_.size();
+        if (n_pending) *n_pending = (int)pending_.size();
+        return 0;
+    }
+
+    int poll(long long ticket, float* info, float* d_dst) override {
+        auto it = all_.find(ticket);
+        if (it == all_.end()) return -1;
+        EJob& j = *it->second;
+        if (j.state == J_FAILED) { all_.erase(it); return -1; }
+        if (j.state != J_DONE) return 0;
+        if (info) { info[0] = j.info[0]; info[1] = j.info[1]; }
+        if (d_dst) {
+            if (j.result) CILREG_CK(hipMemcpyAsync(d_dst, j.result, j.n * sizeof(float), hipMemcpyDeviceToDevice, s_));
+            CILREG_CK(hipStreamSynchronize(s_));
+        }
+        used_ -= (double)j.bytes;
+        all_.erase(it);
+        return 1;
+    }
+
+    int run_batch(cilreg_job* jobs, int n) override {
+        budget_ = 1e300;
+        std::vector<long long> tickets(n);
+        for (int i = 0; i < n; ++i)
+            if (submit(jobs[i], &tickets[i]) != 0) return -1;
+        if (admit() != 0) return -1;
+        if (!active_.empty()) {
+            int end = 0;
+            for (EJob* j : active_) end = std::max(end, j->start_level + std::max(j->pub.iter, 0));
+            if (build_tables() != 0) return -1;
+            if (opts_.stream_launches) {
+                if (launch_levels(level_, end, false) != 0) return -1;
+            } else {
+                hipGraph_t graph = nullptr;
+                hipGraphExec_t exec = nullptr;
+                CILREG_CK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+                const int rc = launch_levels(level_, end, true);
+                hipError_t e = hipStreamEndCapture(s_, &graph);
+                if (rc != 0 || e != hipSuccess) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    set_error("batch graph capture failed: %s", hipGetErrorString(e));
+                    return -1;
+                }
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e == hipSuccess) e = hipGraphLaunch(exec, s_);
+                if (exec) (void)hipGraphExecDestroy(exec);
+                (void)hipGraphDestroy(graph);
+                if (e != hipSuccess) { set_error("batch graph launch failed: %s", hipGetErrorString(e)); return -1; }
+            }
+            level_ = end;
+            CILREG_CK(hipStreamSynchronize(s_));
+            if (retire() != 0) return -1;
+        }
+        for (int i = 0; i < n; ++i) {
+            EJob& j = *all_[tickets[i]];
+            if (j.state != J_DONE) { set_error("batch job %d did not complete", i); return -1; }
+            jobs[i].infovector[0] = j.info[0];
+            jobs[i].infovector[1] = j.info[1];
+            if (jobs[i].d_output && j.result)
+                CILREG_CK(hipMemcpyAsync(jobs[i].d_output, j.result, j.n * sizeof(float), hipMemcpyDeviceToDevice,
+                                         s_));
+        }
+        CILREG_CK(hipStreamSynchronize(s_));
+        for (int i = 0; i < n; ++i) poll(tickets[i], nullptr, nullptr);
+        return 0;
+    }
+
+private:
+    using T = float;
+    int dev_;
+    double budget_, used_ = 0;
+    Opts opts_;
+    hipStream_t s_;
+    int level_ = 0;
+    long long next_ticket_ = 0;
+    std::map<long long, std::unique_ptr<EJob>> all_;
+    std::deque<EJob*> pending_;
+    std::vector<EJob*> active_;
+    T* mult_ = nullptr;
+    int mult_cap_ = 0;
+    // One method group: its jobs (latest-ending first), their serialised
+    // descriptors (host copy and device table), first-block offsets, end
+    // levels, and its slice [fin_off, fin_off + fin_n) of the finaliser table.
+    struct Table { void* dev = nullptr; size_t cap = 0; std::vector<unsigned char> host; int njobs = 0;
+                   std::vector<int> ends, offsets; std::vector<EJob*> jobs; int fin_off = 0, fin_n = 0; };
+    std::map<GroupKey, Table> tables_;
+    FinJob* fin_dev_ = nullptr;
+    size_t fin_cap_ = 0;
+    int nfin_ = 0;
+    std::vector<hipStream_t> side_;
+    std::vector<hipEvent_t> events_;
+
+    // Reserve every buffer of the job (nothing is allocated yet): method state
+    // in fld order, control block, per-block partials, and for a generated
+    // input the input itself and its shape table.
+    void plan_job(EJob& j) {
+        j.ar = std::make_unique<Arena>();
+        Arena& ar = *j.ar;
+        const size_t nT = j.n * sizeof(T), nS = j.n * sizeof(S);
+        const int d = j.dim, nq = d == 3 ? 6 : 3;
+        int m = 0;
+        auto res = [&](size_t b) { j.off_fld[m++] = ar.reserve(b); };
+        switch (j.pub.method) {
+            case CILREG_ROF: res(nT); res(nT); break;                                   // U x2
+            case CILREG_FGP: for (int i = 0; i < 3 * d; ++i) res(nS); res(nT); res(nT); break;   // P x3 sets, D x2
+            case CILREG_PD: res(nT); res(nT); for (int i = 0; i < 2 * d; ++i) res(nS); res(nT); break;  // U x2, P x2, Ubrk
+            default:                                                                       // U x2, (V, P, Q) x2
+                res(nT); res(nT);
+                for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
+                break;
+        }
+        j.nfld = m;
+        j.off_ctrl = ar.reserve(sizeof(Ctrl));
+        j.off_part = ar.reserve((size_t)j.g.nblocks * 2 * sizeof(double));
+        if (!j.pub.d_input) {
+            j.off_f = ar.reserve(nT);
+            j.off_sh = ar.reserve(sizeof(Shape) * (size_t)std::max(j.pub.phantom_shapes, 1));
+        }
+    }
+
+    int ensure_mult(int iters) {
+        if (iters <= mult_cap_) return 0;
+        const int cap = std::max(iters, 2 * mult_cap_);
+        std::vector<T> h(cap);
+        T tk = 1, m = 0;
+        for (int k = 0; k < cap; ++k) {   // FGP_TV_core.c:138,159 (the single solver's sequence)
+            h[k] = m;
+            const T tkp1 = ((T)1 + dsqrt_host(((T)1 + (T)4 * tk * tk))) * (T)0.5;
+            m = (tk - (T)1) / tkp1;
+            tk = tkp1;
+        }
+        T* d = (T*)pool_alloc(cap * sizeof(T));
+        if (!d) { set_error("mult table allocation failed"); return -1; }
+        CILREG_CK(hipMemcpy(d, h.data(), cap * sizeof(T), hipMemcpyHostToDevice));
+        if (mult_) pool_free(mult_);
+        mult_ = d;
+        mult_cap_ = cap;
+        return 0;
+    }
+
+    // allocate and initialise one job's device state
+    int alloc_job(EJob& j) {
+        Arena& ar = *j.ar;
+        const size_t nT = j.n * sizeof(T), nS = j.n * sizeof(S);
+        const int d = j.dim, nq = d == 3 ? 6 : 3;
+        if (ar.allocate() != 0) return -1;
+        for (int i = 0; i < j.nfld; ++i) j.fld[i] = ar.at(j.off_fld[i]);
+        j.ctrl = (Ctrl*)ar.at(j.off_ctrl);
+        j.partials = (double*)ar.at(j.off_part);
+        launch_ctrl_reset(j.ctrl, s_);
+        if (j.pub.d_input) {
+            j.f = j.pub.d_input;
+        } else {
+            j.f = (const float*)ar.at(j.off_f);
+            j.shapes = (Shape*)ar.at(j.off_sh);
+            if (launch_phantom((float*)j.f, j.pub.dimX, j.pub.dimY, j.nz, j.pub.phantom_shapes, j.pub.phantom_sigma,
+                               j.pub.phantom_seed, j.shapes, s_) != 0)
+                return -1;
+        }
+        // initial state, as the single solvers
+        switch (j.pub.method) {
+            case CILREG_ROF: launch_copy(j.fld[0], j.f, nT, s_); break;
+            case CILREG_FGP:
+                for (int b = 0; b < 3; b += 2)
+                    for (int c = 0; c < d; ++c) launch_zero(j.fld[b * d + c], nS, s_);
+                if (j.pub.epsil != 0.0f) {   // D_{-1} = caller's output (FGP_TV_core.c:66,123)
+                    if (j.pub.d_output) launch_copy(j.fld[3 * d + 1], j.pub.d_output, nT, s_);
+                    else launch_zero(j.fld[3 * d + 1], nT, s_);
+                }
+                if (ensure_mult(j.pub.iter) != 0) return -1;
+                break;
+            case CILREG_PD:
+                launch_copy(j.fld[0], j.f, nT, s_);
+                for (int c = 0; c < d; ++c) launch_zero(j.fld[2 + c], nS, s_);
+                break;
+            default:
+                launch_copy(j.fld[0], j.f, nT, s_);
+                for (int i = 0; i < 2 * d + nq; ++i) launch_zero(j.fld[2 + i], nS, s_);   // set 0 of V, P, Q
+                break;
+        }
+        return 0;
+    }
+
+    int admit() {
+        while (!pending_.empty()) {
+            EJob* j = pending_.front();
+            if ((double)j->bytes > budget_) {
+                pending_.pop_front();
+                j->state = J_FAILED;
+                set_error("job %lld needs %zu bytes, above the budget", j->ticket, j->bytes);
+                continue;
+            }
+            if (used_ + (double)j->bytes > budget_) break;
+            pending_.pop_front();
+            if (alloc_job(*j) != 0) { j->state = J_FAILED; return -1; }
+            used_ += (double)j->bytes;
+            j->start_level = level_;
+            j->state = J_ACTIVE;
+            if (j->pub.iter <= 0) {   // nothing to iterate: ROF/PD/SB/TGV return the input, FGP its output
+                j->state = J_DONE;
+                j->result = j->pub.method == CILREG_FGP ? nullptr : j->f;
+                continue;
+            }
+            active_.push_back(j);
+        }
+        return 0;
+    }
+
+    GroupKey key_of(const EJob& j) const {
+        const int iso = (j.pub.method == CILREG_FGP || j.pub.method == CILREG_PD) ? (j.pub.methodTV == 0) : 1;
+        return GroupKey{j.pub.method, j.dim, iso, j.wide ? 1 : 0};
+    }
+
+    template <class J>
+    void fill_hdr(J& d, EJob& j, int offset) {
+        d.h.g = j.g;
+        d.h.block_offset = offset;
+        d.h.start_level = j.start_level;
+        d.h.iters = j.pub.iter;
+        d.h.check_every = j.pub.epsil != 0.0f ? opts_.check_every : 0;
+        d.h.ctrl = j.ctrl;
+        d.h.partials = j.partials;
+    }
+
+    // serialise one job's descriptor for its group
+    void describe(EJob& j, int offset, std::vector<unsigned char>& out) {
+        const int d = j.dim, nq = d == 3 ? 6 : 3;
+        auto put = [&](const void* p, size_t sz) {
+            const size_t at = out.size();
+            out.resize(at + sz);
+            std::memcpy(out.data() + at, p, sz);
+        };
+        auto ch = [&](int base, int c, int nch) { return j.fld[base + (c < nch ? c : 0)]; };
+        switch (j.pub.method) {
+            case CILREG_ROF: {
+                RofJob<T> r{};
+                fill_hdr(r, j, offset);
+                r.f = j.f;
+                r.U[0] = (T*)j.fld[0];
+                r.U[1] = (T*)j.fld[1];
+                r.lambda = j.pub.lambda;
+                r.tau = j.pub.tau;
+                put(&r, sizeof r);
+                break;
+            }
+            case CILREG_FGP: {
+                FgpJob<T, S> r{};
+                fill_hdr(r, j, offset);
+                r.f = j.f;
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) r.P[b][c] = (S*)ch(b * d, c, d);
+                r.D[0] = (T*)j.fld[3 * d];
+                r.D[1] = (T*)j.fld[3 * d + 1];
+                r.mult = mult_;
+                r.lambda = j.pub.lambda;
+                r.step = (T)1 / ((d == 3 ? (T)12 : (T)8) * j.pub.lambda);
+                r.nonneg = j.pub.nonneg != 0;
+                put(&r, sizeof r);
+                break;
+            }
+            case CILREG_PD: {
+                PdJob<T, S> r{};
+                fill_hdr(r, j, offset);
+                r.f = j.f;
+                r.U[0] = (T*)j.fld[0];
+                r.U[1] = (T*)j.fld[1];
+                for (int b = 0; b < 2; ++b)
+                    for (int c = 0; c < 3; ++c) r.P[b][c] = (S*)ch(2 + b * d, c, d);
+                r.Ubrk = (T*)j.fld[2 + 2 * d];
+                const T tau = j.pub.lambda * (T)0.1;   // PD_TV_core.c:50-53
+                r.tau = tau;
+                r.sigma = (T)(1.0 / (double)(j.pub.lipschitz * tau));
+                r.lt = tau / j.pub.lambda;
+                r.nonneg = j.pub.nonneg != 0;
+                put(&r, sizeof r);
+                break;
+            }
+            default: {
+                TgvJob<T, S> r{};
+                fill_hdr(r, j, offset);
+                r.f = j.f;
+                r.U[0] = (T*)j.fld[0];
+                r.U[1] = (T*)j.fld[1];
+                const int per = 2 * d + nq;   // V(d), P(d), Q(nq) per set
+                for (int b = 0; b < 2; ++b) {
+                    const int base = 2 + b * per;
+                    for (int c = 0; c < 3; ++c) {
+                        r.V[b][c] = (S*)ch(base, c, d);
+                        r.P[b][c] = (S*)ch(base + d, c, d);
+                    }
+                    for (int c = 0; c < 6; ++c) r.Q[b][c] = (S*)ch(base + 2 * d, c, nq);
+                }
+                r.lambda = j.pub.lambda;
+                r.tau = (T)pow((double)j.pub.L2, -0.5);   // TGV_core.c:54-55
+                r.alpha1 = j.pub.alpha1;
+                r.alpha0 = j.pub.alpha0;
+                put(&r, sizeof r);
+                break;
+            }
+        }
+    }
+
+    int build_tables() {
+        for (auto& kv : tables_) { kv.second.jobs.clear(); kv.second.host.clear(); kv.second.njobs = 0; }
+        for (EJob* j : active_) tables_[key_of(*j)].jobs.push_back(j);
+        std::vector<FinJob> fins;
+        for (auto& kv : tables_) {
+            Table& t = kv.second;
+            // jobs ending last first, so the jobs still running at any level are a prefix
+            std::stable_sort(t.jobs.begin(), t.jobs.end(), [](const EJob* a, const EJob* b) {
+                return a->start_level + a->pub.iter > b->start_level + b->pub.iter;
+            });
+            t.ends.clear();
+            t.offsets.clear();
+            t.fin_off = (int)fins.size();
+            int off = 0;
+            for (EJob* j : t.jobs) {
+                describe(*j, off, t.host);
+                t.offsets.push_back(off);
+                t.ends.push_back(j->start_level + j->pub.iter);
+                off += j->g.nblocks;
+                if (j->pub.epsil != 0.0f)
+                    fins.push_back(FinJob{j->ctrl, j->partials, j->g.nblocks, j->start_level, j->pub.iter,
+                                          opts_.check_every, j->pub.epsil});
+            }
+            t.offsets.push_back(off);
+            t.fin_n = (int)fins.size() - t.fin_off;
+            t.njobs = (int)t.jobs.size();
+            if (t.host.size() > t.cap) {
+                if (t.dev) pool_free(t.dev);
+                t.cap = std::max(t.host.size(), 2 * t.cap);
+                t.dev = pool_alloc(t.cap);
+                if (!t.dev) { set_error("job table allocation failed"); return -1; }
+            }
+            if (!t.host.empty())
+                CILREG_CK(hipMemcpyAsync(t.dev, t.host.data(), t.host.size(), hipMemcpyHostToDevice, s_));
+        }
+        nfin_ = (int)fins.size();
+        if (nfin_) {
+            if ((size_t)nfin_ * sizeof(FinJob) > fin_cap_) {
+                if (fin_dev_) pool_free(fin_dev_);
+                fin_cap_ = std::max((size_t)nfin_ * sizeof(FinJob), 2 * fin_cap_);
+                fin_dev_ = (FinJob*)pool_alloc(fin_cap_);
+                if (!fin_dev_) { set_error("finaliser table allocation failed"); return -1; }
+            }
+            CILREG_CK(hipMemcpyAsync(fin_dev_, fins.data(), nfin_ * sizeof(FinJob), hipMemcpyHostToDevice, s_));
+        }
+        // the host tables must outlive the asynchronous copies
+        CILREG_CK(hipStreamSynchronize(s_));
+        return 0;
+    }
+
+    template <int DIM, int TY>
+    void launch_group(const GroupKey& k, const Table& t, int nact, int total, int level, hipStream_t st) {
+        const dim3 grid(total), block(TX, TY);
+        const bool w = k.wide != 0, iso = k.iso != 0;
+        switch (k.method) {
+            case CILREG_ROF:
+                if (w) hipLaunchKernelGGL((rof_group_kernel<DIM, TY, T, uint64_t>), grid, block, 0, st, (const RofJob<T>*)t.dev, nact, total, level);
+                else hipLaunchKernelGGL((rof_group_kernel<DIM, TY, T, uint32_t>), grid, block, 0, st, (const RofJob<T>*)t.dev, nact, total, level);
+                break;
+            case CILREG_FGP: {
+                auto* jp = (const FgpJob<T, S>*)t.dev;
+                if (w) {
+                    if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                } else {
+                    if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                }
+                break;
+            }
+            case CILREG_PD: {
+                auto* jp = (const PdJob<T, S>*)t.dev;
+                if (w) {
+                    if (iso) hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                } else {
+                    if (iso) hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                }
+                break;
+            }
+            default: {
+                auto* jp = (const TgvJob<T, S>*)t.dev;
+                if (w) hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                else hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                break;
+            }
+        }
+    }
+
+    // Launch levels [l0, l1) of every group.  Groups run on their own streams
+    // (forked from and joined back to the main stream with events, which also
+    // works under stream capture), so different methods overlap.
+    int launch_levels(int l0, int l1, bool capturing) {
+        std::vector<std::pair<GroupKey, Table*>> groups;
+        for (auto& kv : tables_)
+            if (kv.second.njobs > 0) groups.push_back({kv.first, &kv.second});
+        const int ng = (int)groups.size();
+        while ((int)side_.size() < ng) {
+            hipStream_t st;
+            CILREG_CK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            side_.push_back(st);
+        }
+        while ((int)events_.size() < ng + 1) {
+            hipEvent_t e;
+            CILREG_CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            events_.push_back(e);
+        }
+        CILREG_CK(hipEventRecord(events_[ng], s_));
+        for (int gi = 0; gi < ng; ++gi) CILREG_CK(hipStreamWaitEvent(side_[gi], events_[ng], 0));
+        for (int gi = 0; gi < ng; ++gi) {
+            const GroupKey& k = groups[gi].first;
+            const Table& t = *groups[gi].second;
+            hipStream_t st = side_[gi];
+            for (int lv = l0; lv < l1; ++lv) {
+                int nact = 0;
+                while (nact < t.njobs && t.ends[nact] > lv) ++nact;
+                if (nact == 0) break;
+                const int total = t.offsets[nact];
+                if (k.dim == 3) launch_group<3, 8>(k, t, nact, total, lv, st);
+                else launch_group<2, 4>(k, t, nact, total, lv, st);
+                if (t.fin_n)
+                    hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st,
+                                       (const FinJob*)fin_dev_ + t.fin_off, lv);
+            }
+            CILREG_CK(hipEventRecord(events_[gi], st));
+            CILREG_CK(hipStreamWaitEvent(s_, events_[gi], 0));
+        }
+        (void)capturing;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { set_error("group launch failed: %s", hipGetErrorString(e)); return -1; }
+        return 0;
+    }
+
+    int retire() {
+        std::vector<Ctrl> c(active_.size());
+        for (size_t i = 0; i < active_.size(); ++i)
+            CILREG_CK(hipMemcpyAsync(&c[i], active_[i]->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s_));
+        CILREG_CK(hipStreamSynchronize(s_));
+        std::vector<EJob*> still;
+        for (size_t i = 0; i < active_.size(); ++i) {
+            EJob& j = *active_[i];
+            const int iters = j.pub.iter;
+            const bool stopped = c[i].stopped != 0;
+            if (!stopped && level_ < j.start_level + iters) { still.push_back(&j); continue; }
+            const int done = stopped ? c[i].stop_iter : iters;
+            const int last = stopped ? done : iters - 1;
+            const int d = j.dim;
+            switch (j.pub.method) {
+                case CILREG_ROF: j.result = (const float*)j.fld[(last + 1) & 1]; break;
+                case CILREG_FGP: j.result = (const float*)j.fld[3 * d + (last & 1)]; break;
+                case CILREG_PD: j.result = stopped ? (const float*)j.fld[2 + 2 * d] : (const float*)j.fld[iters & 1]; break;
+                default: j.result = (const float*)j.fld[(last + 1) & 1]; break;
+            }
+            j.info[0] = (float)done;
+            j.info[1] = c[i].re;
+            j.state = J_DONE;
+        }
+        active_.swap(still);
+        return 0;
+    }
+};
+
+static EngineBase* make_engine(int dev, double budget, const Opts& o, hipStream_t s) {
+    if (o.precision == CILREG_BF16) return new Engine<bf16_t>(dev, budget, o, s);
+    if (o.precision == CILREG_FP16) return new Engine<_Float16>(dev, budget, o, s);
+    return new Engine<float>(dev, budget, o, s);
+}
+
+}  // namespace cilreg
+
+using namespace cilreg;
+
+struct cilreg_server {
+    std::unique_ptr<EngineBase> eng;
+    int dev;
+    hipStream_t stream;
+    std::mutex mu;
+};
+
+extern "C" {
+
+int cilreg_phantom_dev(float* d_out, int dimX, int dimY, int dimZ, int n_shapes, float sigma, unsigned long long seed,
+                       void* stream) {
+    if (dimX <= 0 || dimY <= 0 || n_shapes < 0) { set_error("invalid phantom arguments"); return -1; }
+    hipStream_t s = caller_stream(stream);
+    if (!s) { set_error("no stream"); return -1; }
+    Arena ar;
+    const size_t o = ar.reserve(sizeof(Shape) * (size_t)(n_shapes > 0 ? n_shapes : 1));
+    if (ar.allocate() != 0) return -1;
+    return launch_phantom(d_out, dimX, dimY, dimZ, n_shapes, sigma, seed, (Shape*)ar.at(o), s);
+}
+
+int cilreg_batch_run(cilreg_job* jobs, int njobs, const cilreg_opts* opts, void* stream) {
+    if (njobs < 0 || (njobs > 0 && !jobs)) { set_error("invalid batch"); return -1; }
+    int dev = 0;
+    CILREG_CK(hipGetDevice(&dev));
+    hipStream_t s = caller_stream(stream);
+    if (!s) { set_error("no stream"); return -1; }
+    std::unique_ptr<EngineBase> eng(make_engine(dev, 1e300, resolve_opts(opts), s));
+    return eng->run_batch(jobs, njobs);
+}
+
+cilreg_server* cilreg_server_create(int gpu_device, double hbm_budget_bytes, const cilreg_opts* opts) {
+    if (select_device(gpu_device) != 0) return nullptr;
+    auto* srv = new cilreg_server();
+    srv->dev = gpu_device;
+    if (hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("cannot create the server stream");
+        delete srv;
+        return nullptr;
+    }
+    srv->eng.reset(make_engine(gpu_device, hbm_budget_bytes, resolve_opts(opts), srv->stream));
+    return srv;
+}
+
+int cilreg_server_submit(cilreg_server* srv, const cilreg_job* job, long long* ticket) {
+    if (!srv || !job) return -1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    CILREG_CK(hipSetDevice(srv->dev));
+    if (job->d_input) {   // the input may still be in flight on the legacy default stream
+        hipEvent_t e = nullptr;
+        CILREG_CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const hipError_t r1 = hipEventRecord(e, nullptr), r2 = hipStreamWaitEvent(srv->stream, e, 0);
+        (void)hipEventDestroy(e);
+        if (r1 != hipSuccess || r2 != hipSuccess) { set_error("cannot order the server stream"); return -1; }
+    }
+    return srv->eng->submit(*job, ticket);
+}
+
+int cilreg_server_step(cilreg_server* srv, int levels, int* n_active, int* n_pending) {
+    if (!srv) return -1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    CILREG_CK(hipSetDevice(srv->dev));
+    return srv->eng->step(levels, n_active, n_pending);
+}
+
+int cilreg_server_poll(cilreg_server* srv, long long ticket, float* infovector, float* d_dst) {
+    if (!srv) return -1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    CILREG_CK(hipSetDevice(srv->dev));
+    return srv->eng->poll(ticket, infovector, d_dst);
+}
+
+void cilreg_server_destroy(cilreg_server* srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        (void)hipSetDevice(srv->dev);
+        (void)hipStreamSynchronize(srv->stream);
+        srv->eng.reset();
+        (void)hipStreamDestroy(srv->stream);
+    }
+    delete srv;
+}
+
+}  // extern "C"

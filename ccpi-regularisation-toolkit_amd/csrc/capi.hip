@@ -55,7 +55,7 @@ bool dims_ok(int nx, int ny, int nz) {
 }
 
 hipStream_t pick_stream(void* stream) {
-    return stream ? (hipStream_t)stream : device_stream();
+    return caller_stream(stream);
 }
 
 }  // namespace

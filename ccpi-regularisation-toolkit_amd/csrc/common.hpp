@@ -144,9 +144,9 @@ __device__ inline int xcd_logical_block(int b, int nb) {
 
 struct TileCoord { int x0, y0, z0, z1; int lin; };
 
-__device__ inline TileCoord tile_of_block(const Geom& g, int ty) {
+// Tile of logical block L (tiles in x fastest, then y, then z chunks).
+__device__ inline TileCoord tile_of_logical(const Geom& g, int ty, int L) {
     TileCoord t;
-    int L = xcd_logical_block((int)blockIdx.x, g.nblocks);
     t.lin = L;
     int txi = L % g.tiles_x;
     int rem = L / g.tiles_x;
@@ -157,6 +157,10 @@ __device__ inline TileCoord tile_of_block(const Geom& g, int ty) {
     t.z0 = zci * g.zc;
     t.z1 = t.z0 + g.zc < g.nz ? t.z0 + g.zc : g.nz;
     return t;
+}
+
+__device__ inline TileCoord tile_of_block(const Geom& g, int ty) {
+    return tile_of_logical(g, ty, xcd_logical_block((int)blockIdx.x, g.nblocks));
 }
 
 // ---------------------------------------------------------------------------

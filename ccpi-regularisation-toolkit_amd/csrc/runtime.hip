@@ -46,6 +46,20 @@ hipStream_t device_stream() {
     return streams[dev];
 }
 
+hipStream_t caller_stream(void* stream) {
+    if (stream) return (hipStream_t)stream;
+    hipStream_t s = device_stream();
+    if (!s) return nullptr;
+    // The library stream does not synchronise with the legacy default stream:
+    // order it after everything already queued there (e.g. the kernel that
+    // produced the input).
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    const bool ok = hipEventRecord(e, nullptr) == hipSuccess && hipStreamWaitEvent(s, e, 0) == hipSuccess;
+    (void)hipEventDestroy(e);
+    return ok ? s : nullptr;
+}
+
 // ---------------------------------------------------------------------------
 // caching device allocator
 // ---------------------------------------------------------------------------
@@ -182,38 +196,7 @@ int PinnedCtrl::allocate() {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void finalize_kernel(const double* partials, int nblocks, Ctrl* ctrl, float epsil,
                                                        int iter) {
-    __shared__ double red[2 * 256];
-    if (ctrl->stopped) return;
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < nblocks; i += 256) {
-        a += partials[2 * i];
-        b += partials[2 * i + 1];
-    }
-    red[threadIdx.x] = a;
-    red[256 + threadIdx.x] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[threadIdx.x] += red[threadIdx.x + s];
-            red[256 + threadIdx.x] += red[256 + threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double sd = red[0], su = red[256];
-        // re = sqrt(sum d^2) / sqrt(sum u^2), as ROF_TV_core.c:86 (0/0 -> NaN never stops)
-        float re = (float)(sqrt(sd) / sqrt(su));
-        ctrl->re = re;
-        ctrl->sum_d = sd;
-        ctrl->sum_u = su;
-        int count = ctrl->count;
-        if (re < epsil) count++;
-        ctrl->count = count;
-        if (count > 3) {
-            ctrl->stop_iter = iter;
-            ctrl->stopped = 1;
-        }
-    }
+    finalize_body(partials, nblocks, ctrl, epsil, iter);
 }
 
 __global__ void ctrl_reset_kernel(Ctrl* ctrl) {
@@ -284,6 +267,7 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.zchunk = o->zchunk;
         r.profile = o->profile;
         r.wide_index = o->wide_index;
+        r.stream_launches = o->stream_launches;
     }
     return r;
 }

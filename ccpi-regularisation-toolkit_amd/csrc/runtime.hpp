@@ -32,6 +32,9 @@ int select_device(int dev);
 
 // The library's own stream on the current device (default priority, created once).
 hipStream_t device_stream();
+// The stream a device-API call runs on: the caller's, or for NULL the
+// library's stream ordered after the work queued on the legacy default stream.
+hipStream_t caller_stream(void* stream);
 
 // Caching device allocator: blocks freed by a finished solve are kept and
 // handed to the next solve that fits (at most 2x its request), so repeated
@@ -68,6 +71,45 @@ public:
 
 // Launch the fixed-order reduction of per-block partials that applies the
 // stopping rule for iteration `iter` (one block; deterministic).
+// The stopping rule's final reduction for one solve, run by one 256-thread
+// workgroup: sums the per-block partials in a fixed order, stores the
+// residual and applies the rule.  Shared by the single-solve finaliser and the
+// grouped one, so both reach the same decision from the same partials.
+__device__ inline void finalize_body(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter) {
+    __shared__ double red[2 * 256];
+    if (ctrl->stopped) return;
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    red[threadIdx.x] = a;
+    red[256 + threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[threadIdx.x] += red[threadIdx.x + s];
+            red[256 + threadIdx.x] += red[256 + threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sd = red[0], su = red[256];
+        // re = sqrt(sum d^2) / sqrt(sum u^2), as ROF_TV_core.c:86 (0/0 -> NaN never stops)
+        float re = (float)(sqrt(sd) / sqrt(su));
+        ctrl->re = re;
+        ctrl->sum_d = sd;
+        ctrl->sum_u = su;
+        int count = ctrl->count;
+        if (re < epsil) count++;
+        ctrl->count = count;
+        if (count > 3) {
+            ctrl->stop_iter = iter;
+            ctrl->stopped = 1;
+        }
+    }
+}
+
 void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s);
 // Zero the control block (a small fill kernel, safe under graph capture).
 void launch_ctrl_reset(Ctrl* ctrl, hipStream_t s);
@@ -87,6 +129,7 @@ struct Opts {
     int poll = 16;         // host polls the stop flag every `poll` iterations
     int profile = 0;       // time each iteration with HIP events
     int wide_index = 0;    // force 64-bit element indexing
+    int stream_launches = 0;  // batch: launch directly instead of one hipGraph
 };
 
 // Per-iteration kernel timing of the last profiled solve on this thread.

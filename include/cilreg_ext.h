@@ -12,7 +12,8 @@
  *  - cilreg_batch_*: many independent volumes in one launch sequence;
  *  - cilreg_last_error: the thread's last error message.
  * All pointers named d_* are device pointers on the current device; `stream`
- * is a hipStream_t (NULL = the library's own stream).  Every call returns 0 on
+ * is a hipStream_t (NULL = the library's own stream, ordered after the work
+ * already queued on the legacy default stream).  Every call returns 0 on
  * success and -1 on error, and is synchronous at return.
  */
 #ifndef CILREG_EXT_H
@@ -39,8 +40,35 @@ typedef struct cilreg_opts {
     int zchunk;          /* z planes marched per workgroup; 0 = auto */
     int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
     int wide_index;      /* 1: force 64-bit element indexing (chosen automatically for fields >= 2 GiB) */
-    int reserved[2];
+    int stream_launches; /* batch: 1 = launch directly instead of capturing one hipGraph */
+    int reserved;
 } cilreg_opts;
+
+/* Methods of a batch or served job. */
+enum { CILREG_ROF = 0, CILREG_FGP = 1, CILREG_PD = 2, CILREG_TGV = 3 };
+
+/* One volume of a batch or one request of the server.  Parameters not used
+ * by the method are ignored.  The input is either a device array (d_input) or,
+ * when d_input is NULL and phantom_shapes > 0, a seeded ellipsoid phantom
+ * generated on the device (cilreg_phantom_dev). */
+typedef struct cilreg_job {
+    int method;                    /* CILREG_ROF / _FGP / _PD / _TGV */
+    int dimX, dimY, dimZ;          /* dimX fastest; dimZ <= 1: 2D */
+    int iter;                      /* iteration count */
+    float epsil;                   /* tolerance, 0 = fixed iteration count */
+    float lambda;                  /* regularisation parameter (all methods) */
+    float tau;                     /* ROF marching step */
+    float lipschitz;               /* PD Lipschitz constant (<= 0: 8, as in demos/demo_gpu_regularisers3D.py:197) */
+    float alpha1, alpha0, L2;      /* TGV (L2 <= 0: 12, as in demos/demo_cpu_regularisers3D.py:338) */
+    int methodTV;                  /* FGP, PD: 0 isotropic, 1 anisotropic */
+    int nonneg;                    /* FGP, PD */
+    const float *d_input;          /* device input, or NULL for a generated phantom */
+    float *d_output;               /* device output (may be NULL for server jobs: see cilreg_server_poll) */
+    int phantom_shapes;            /* generated input: number of ellipsoids */
+    float phantom_sigma;           /* generated input: Gaussian noise sigma */
+    unsigned long long phantom_seed;
+    float infovector[2];           /* out: [iterations, last residual] */
+} cilreg_job;
 
 /* Last error recorded on this host thread ("" if none). */
 CILREG_API const char *cilreg_last_error(void);
@@ -81,6 +109,34 @@ CILREG_API int cilreg_fgp_dev_f64(const double *d_input, double *d_output, doubl
 CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, double *infovector, double lambda, double alpha1,
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts *opts, void *stream);
+
+/* Runs every job of the batch to completion with grouped kernels: one launch
+ * per iteration level and method group covers all volumes of that group, so
+ * many small or uneven volumes fill the device together.  Unless
+ * opts->stream_launches is set the whole level sequence is captured into one
+ * hipGraph (one parallel branch per method group) and launched once.
+ * Each volume's result is bit-identical to solving it alone. */
+CILREG_API int cilreg_batch_run(cilreg_job *jobs, int njobs, const cilreg_opts *opts, void *stream);
+
+/* Seeded synthetic input on the device: n_shapes random axis-aligned
+ * ellipsoids (3D) or rectangles/ellipses (2D), max-combined, plus N(0,
+ * sigma^2) noise from a counter-based generator (deterministic for a seed). */
+CILREG_API int cilreg_phantom_dev(float *d_out, int dimX, int dimY, int dimZ, int n_shapes, float sigma,
+                                  unsigned long long seed, void *stream);
+
+/* Continuous-batching server.  Jobs are admitted while their device memory
+ * fits the HBM budget; each step runs `levels` iterations of every admitted
+ * job with grouped kernels (jobs at different iteration counts share the same
+ * launches), then retires finished jobs.  State stays on the device between
+ * steps.  A job's result is bit-identical to solving it alone. */
+typedef struct cilreg_server cilreg_server;
+CILREG_API cilreg_server *cilreg_server_create(int gpu_device, double hbm_budget_bytes, const cilreg_opts *opts);
+CILREG_API int cilreg_server_submit(cilreg_server *srv, const cilreg_job *job, long long *ticket);
+CILREG_API int cilreg_server_step(cilreg_server *srv, int levels, int *n_active, int *n_pending);
+/* 1 when done (infovector filled; the output copied to d_dst if not NULL, then
+ * the job's memory is released), 0 when pending or running, -1 if unknown. */
+CILREG_API int cilreg_server_poll(cilreg_server *srv, long long ticket, float *infovector, float *d_dst);
+CILREG_API void cilreg_server_destroy(cilreg_server *srv);
 
 #ifdef __cplusplus
 }

@@ -105,3 +105,24 @@ def test_opts_struct_matches_header():
     nints = sum(int(m.group(1)) if m.group(1) else 1
                 for m in re.finditer(r"int\s+\w+(?:\[(\d+)\])?;", body))
     assert ctypes.sizeof(CilregOpts) == 4 * nints
+
+
+def test_job_struct_matches_header(tmp_path):
+    # the ctypes mirror of cilreg_job against the C compiler's layout
+    import shutil
+    import subprocess
+    from ccpi.filters.batch import CilregJob
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f[0] for f in CilregJob._fields_]
+    cnames = ["lambda" if f == "lambda_" else f for f in fields]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cilreg_ext.h"\nint main(void){\n'
+                   '  printf("%zu\\n", sizeof(cilreg_job));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(cilreg_job, {c}));\n' for c in cnames) + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(CilregJob)
+    assert got[1:] == [getattr(CilregJob, f).offset for f in fields]
