@@ -8,18 +8,28 @@ checked every iteration.  One step = one full solve with the input already in
 HBM; value = voxel-iterations per second over the timed steps.
 
 Also reported:
-* roofline: the fused FGP iteration kernel's algorithmic bytes per launch
-  (48 B/voxel with the every-iteration residual check) over its mean launch
-  time, measured with HIP events on the solve's stream around every
-  iteration's kernel;
+* roofline: the FGP kernel's algorithmic bytes (48 B per voxel-iteration with
+  the every-iteration residual check; a launch advances two iterations) over
+  its launch time, measured with HIP events on the solve's stream around every
+  launch;
 * cpu_baseline: the reference C-OMP library (oracle/_ref, built from the
   reference sources) or, where it is absent, the C restatement (oracle/), on a
   bounded sample of the same workload, on the host's cores;
-* secondary: TGV 3D 192x512x512 (500 iterations, FP32), PD-TV 3D 256^3
-  (400 iterations, L = 12) and ROF-TV 2D 512^2 (2000 iterations) solves.
+* secondary: TGV 3D 192x512x512 (500 iterations; FP32, BF16 and FP16 dual
+  storage against an FP64 solve), PD-TV 3D 256^3 (400 iterations, L = 12),
+  ROF-TV 2D 512^2 (2000 iterations); 64 uneven volumes (half FGP, half ROF)
+  solved one after another vs in one batched hipGraph launch (FP32 and BF16
+  duals); a Poisson request stream (200 req/s, ROF/FGP/PD/TGV, 2D and 3D)
+  replayed in real time through the continuous-batching server (p50/p99
+  latency, aggregate voxel-iterations/s) vs the same requests solved plainly.
+  The arrival window defaults to 0.5 s instead of 60 s: the offered load
+  exceeds one device's throughput, so the full window would take many
+  minutes to drain (--workload serve --serve-window 60 runs it).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
-       python bench.py --workload {tgv3d,pd3d,copy} --steps K   (single-workload runs for
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu] [--no-batch]
+                       [--no-serve] [--serve-window SECONDS]
+       python bench.py --profile-pass --steps K --warmup 0   (the headline solves only)
+       python bench.py --workload {tgv3d,pd3d,copy,batch,serve} --steps K   (single-workload runs for
        profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
        known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -28,6 +38,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ccpi-regularisation-toolkit_amd"))
@@ -179,6 +191,171 @@ def secondary(D, torch):
     return out
 
 
+def batched_jobs(seed=4):
+    """64 uneven ellipsoid-phantom volumes, each extent drawn from U{64..320},
+    noise sigma U[0.02, 0.08]; even entries FGP-TV (lambda U[0.01, 0.05], 300
+    iterations, isotropic), odd entries ROF-TV (lambda U[0.01, 0.05], time
+    step 0.001, 1000 iterations); fixed iteration counts."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(64):
+        shape = tuple(int(v) for v in rng.integers(64, 321, size=3))
+        sigma = float(rng.uniform(0.02, 0.08))
+        lam = float(rng.uniform(0.01, 0.05))
+        if i % 2 == 0:
+            spec = {"method": "fgp", "iters": 300, "lambda": lam, "method_tv": 0, "nonneg": 0}
+        else:
+            spec = {"method": "rof", "iters": 1000, "lambda": lam, "tau": 0.001}
+        jobs.append((shape, (20, sigma, 9000 + i), spec))
+    return jobs
+
+
+def batched_section(D, B, torch):
+    """Plain (one solve after another) vs one batched hipGraph launch, FP32 and BF16 duals."""
+    jobs = batched_jobs()
+    xs = [B.phantom(shape, *ph) for shape, ph, _ in jobs]
+    specs = [dict(sp, input=x) for (_, _, sp), x in zip(jobs, xs)]
+    total_vi = sum(x.numel() * sp["iters"] for x, sp in zip(xs, specs))
+
+    def plain(specs, prec):
+        outs = []
+        for sp in specs:
+            x = sp["input"]
+            if sp["method"] == "fgp":
+                outs.append(D.fgp(x, sp["lambda"], sp["iters"], 0.0, 0, 0, precision=prec)[0])
+            else:
+                outs.append(D.rof(x, sp["lambda"], sp["iters"], sp["tau"], 0.0)[0])
+        return outs
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    short = [dict(sp, iters=2) for sp in specs]
+    plain(short, "fp32")   # warm-up: workspaces, code objects
+    B.batch_run(short)
+    B.batch_run(short, precision="bf16")
+    log("bench: batched section, plain FP32")
+    ref, t_plain = timed(lambda: plain(specs, "fp32"))
+    log("bench: batched section, one graph FP32")
+    got32, t_b32 = timed(lambda: B.batch_run(specs))
+    identical = all(bool(torch.equal(a, b)) for a, (b, _) in zip(ref, got32))
+    del got32
+    log("bench: batched section, one graph BF16 duals")
+    got16, t_b16 = timed(lambda: B.batch_run(specs, precision="bf16"))
+    # BF16 error against FP64 solves of the first four FGP volumes (ROF has no dual field)
+    errs = []
+    for sp, (y, _) in list(zip(specs, got16))[:8]:
+        if sp["method"] != "fgp":
+            continue
+        r64, _ = D.fgp_f64(sp["input"].double(), sp["lambda"], sp["iters"], 0.0, 0, 0)
+        errs.append(float(torch.linalg.vector_norm(y.double() - r64) / torch.linalg.vector_norm(r64)))
+        del r64
+    del got16, ref
+    return {
+        "workload": "64 volumes, extents U{64..320}^3, half FGP-TV 300 iterations, half ROF-TV 1000 iterations",
+        "voxel_iters": total_vi,
+        "plain_fp32": {"seconds": round(t_plain, 4), "voxel_iters_per_s": total_vi / t_plain},
+        "batched_graph_fp32": {"seconds": round(t_b32, 4), "voxel_iters_per_s": total_vi / t_b32,
+                               "speedup_vs_plain": t_plain / t_b32, "bit_identical_to_plain": identical},
+        "batched_graph_bf16": {"seconds": round(t_b16, 4), "voxel_iters_per_s": total_vi / t_b16,
+                               "speedup_vs_plain": t_plain / t_b16,
+                               "fgp_rel_l2_vs_fp64_max_of_4": max(errs) if errs else None},
+    }
+
+
+def request_stream(rate=200.0, seconds=60.0, seed=5):
+    """Poisson arrivals; each request ROF/FGP/PD/TGV uniformly, a 2D slice
+    (square, side U{256..2048}) with p = 0.5 else a 3D cube (side U{64..384}),
+    generated phantom with noise sigma U[0.02, 0.08], lambda U[0.01, 0.05],
+    iterations U{100..500}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > seconds:
+            break
+        method = ["rof", "fgp", "pd", "tgv"][int(rng.integers(0, 4))]
+        if rng.random() < 0.5:
+            side = int(rng.integers(256, 2049))
+            shape = (side, side)
+        else:
+            side = int(rng.integers(64, 385))
+            shape = (side, side, side)
+        sigma = float(rng.uniform(0.02, 0.08))
+        spec = {"method": method, "shape": shape, "phantom": (20, sigma, 20000 + len(reqs)),
+                "lambda": float(rng.uniform(0.01, 0.05)), "iters": int(rng.integers(100, 501)),
+                "tau": 0.001, "L": 12.0, "alpha1": 1.0, "alpha0": 2.0, "L2": 12.0}
+        reqs.append((t, spec))
+    return reqs
+
+
+def served_section(D, B, torch, window):
+    """The request stream replayed in real time through the server (FP32 and
+    BF16 duals), and the same requests solved plainly one after another."""
+    import numpy as np
+    reqs = request_stream(seconds=window)
+    out = {"workload": f"Poisson 200 req/s for {window:g} s ({len(reqs)} requests), ROF/FGP/PD/TGV, "
+                       "2D 256^2..2048^2 or 3D 64^3..384^3, 100..500 iterations, HBM budget 200 GB"}
+    # warm-up of code objects and workspaces
+    with B.Server(0, hbm_budget=200e9) as srv:
+        B.serve_stream(srv, [(0.0, dict(sp, iters=2)) for _, sp in reqs[:8]])
+    for prec in ("fp32", "bf16"):
+        log(f"bench: served stream {prec}")
+        with B.Server(0, hbm_budget=200e9, precision=prec) as srv:
+            r = B.serve_stream(srv, reqs, levels=8)
+        lat = np.array(r["latency_s"])
+        vis = np.array(r["voxels"], dtype=np.float64) * np.array(r["iterations"], dtype=np.float64)
+        vi = float(vis.sum())
+        methods = np.array([sp["method"] for _, sp in reqs])
+        per_method = {}
+        for m in ("rof", "fgp", "pd", "tgv"):
+            sel = methods == m
+            if sel.any():
+                per_method[m] = {"requests": int(sel.sum()), "p50_latency_s": float(np.percentile(lat[sel], 50)),
+                                 "p99_latency_s": float(np.percentile(lat[sel], 99)),
+                                 "voxel_iters": float(vis[sel].sum())}
+        out[f"served_{prec}"] = {
+            "requests": len(reqs), "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": r["makespan_s"],
+            "voxel_iters_per_s": vi / r["makespan_s"],
+            "offered_voxel_iters_per_s": vi / window,
+            "mean_jobs_on_device": r["mean_active_per_step"], "server_steps": r["steps"],
+            "per_method": per_method}
+    # plain form: the same requests, one single solve after another (input
+    # generation included, as the server generates it at admission)
+    log(f"bench: served section, the {len(reqs)} requests solved plainly")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vi = 0
+    for _, sp in reqs:
+        x = B.phantom(sp["shape"], *sp["phantom"])
+        m = sp["method"]
+        if m == "rof":
+            _, info = D.rof(x, sp["lambda"], sp["iters"], sp["tau"], 0.0)
+        elif m == "fgp":
+            _, info = D.fgp(x, sp["lambda"], sp["iters"], 0.0, 0, 0)
+        elif m == "pd":
+            _, info = D.pd(x, sp["lambda"], sp["iters"], 0.0, sp["L"], 0, 0)
+        else:
+            _, info = D.tgv(x, sp["lambda"], sp["alpha1"], sp["alpha0"], sp["iters"], sp["L2"], 0.0)
+        vi += x.numel() * int(info[0])
+        del x
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    out["plain_fp32"] = {"requests": len(reqs), "seconds": round(tp, 4), "voxel_iters_per_s": vi / tp}
+    for prec in ("fp32", "bf16"):
+        out[f"served_{prec}"]["vs_plain_voxel_iters"] = (out[f"served_{prec}"]["voxel_iters_per_s"]
+                                                          / out["plain_fp32"]["voxel_iters_per_s"])
+    return out
+
+
 def single_workload(name, steps, D, torch):
     """One workload on its own (for profiler passes); prints a JSON line."""
     from ccpi.supp import phantoms
@@ -219,7 +396,13 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=16)
-    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "copy"])
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only the timed headline solves (for rocprofv3 passes)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-window", type=float, default=0.5,
+                    help="seconds of Poisson arrivals replayed by the served section")
+    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "copy", "batch", "serve"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -232,6 +415,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
+    if args.workload in ("batch", "serve"):
+        from ccpi.filters import batch as B
+        res = batched_section(D, B, torch) if args.workload == "batch" else served_section(D, B, torch,
+                                                                                            args.serve_window)
+        print(json.dumps(res, default=float), flush=True)
+        return
     if args.workload != "fgp3d":
         print(json.dumps(single_workload(args.workload, args.steps, D, torch), default=float), flush=True)
         return
@@ -255,16 +444,40 @@ def main():
     value = total_vi / dt
     log(f"bench: FGP {args.steps} steps in {dt:.4f} s, iterations {iters}, {value:.4e} voxel-iters/s")
 
+    if args.profile_pass:
+        print(json.dumps({"workload": "FGP-TV 3D 256^3 headline solves only", "steps": args.steps,
+                          "iterations_per_step": iters, "voxel_iters_per_s": value}), flush=True)
+        return
+
     # per-kernel timing with HIP events on the solve's stream
     run_fgp(D, x, profile=True)
-    mean_ms, min_ms, cnt = D.last_profile()
-    bytes_per_launch = 48 * n
-    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9 if mean_ms else None
+    mean_ms, min_ms, cnt = D.last_profile()   # per iteration (launch time / 2)
+    per_launch = 2
+    bytes_per_launch = 48 * n * per_launch
+    launch_ms = mean_ms * per_launch if mean_ms else None
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if mean_ms else None
     traffic = read_traffic(n)
+    achievable = D.stream_bench(1 << 30, 20)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": "fgp_iter_kernel<3,8,float,float,iso,uint32>", "kernel_ms_mean": mean_ms,
-                "kernel_ms_min": min_ms, "launches_timed": cnt, "alg_bytes_per_voxel": 48}
+                "kernel": "fgp2_iter_kernel<8,float,float,iso,uint32>", "iterations_per_launch": per_launch,
+                "kernel_ms_mean": launch_ms, "kernel_ms_min": min_ms * per_launch if min_ms else None,
+                "kernel_ms_mean_per_iteration": mean_ms, "iterations_timed": cnt,
+                "alg_bytes_per_voxel_iteration": 48, "alg_bytes_per_launch": bytes_per_launch,
+                "achievable_GBps": achievable,
+                "frac_of_achievable_sum2": achieved / achievable["sum2"] if achieved else None}
+
+    # the same solve through the reference C-ABI on host buffers (PCIe copies
+    # included; the reference's every-5 check cadence)
+    from ccpi.filters import regularisers as R
+    R.FGP_TV(x_np, 0.02, 400, 1e-6, 0, 1, device="gpu")
+    capi_info = np.zeros(2, dtype=np.float32)
+    t0 = time.perf_counter()
+    R.FGP_TV(x_np, 0.02, 400, 1e-6, 0, 1, device="gpu", infovector=capi_info)
+    t_capi = time.perf_counter() - t0
+    capi = {"seconds": round(t_capi, 4), "iterations": int(capi_info[0]),
+            "voxel_iters_per_s": n * float(capi_info[0]) / t_capi,
+            "note": "TV_FGP_GPU_main on host arrays, H2D/D2H included, tolerance checked every 5 iterations"}
 
     cpu = None
     if not args.no_cpu:
@@ -275,6 +488,15 @@ def main():
         del x
         torch.cuda.empty_cache()
         extras = secondary(D, torch)
+        from ccpi.filters import batch as B
+        torch.cuda.empty_cache()
+        if not args.no_batch:
+            extras["batched_uneven_volumes"] = batched_section(D, B, torch)
+            torch.cuda.empty_cache()
+            D.release_cache()
+        if not args.no_serve:
+            extras["served_stream"] = served_section(D, B, torch, args.serve_window)
+            D.release_cache()
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,
@@ -283,7 +505,8 @@ def main():
         "config": {"workload": "FGP-TV 3D 256^3 f32, 40-ellipsoid phantom + N(0,0.05^2), lambda 0.02, "
                                "400 iterations, iso, nonneg, tol 1e-6 checked every iteration",
                    "voxels": n, "iterations_per_step": iters},
-        "hbm_GBps_alg": achieved, "roofline": roofline, "cpu_baseline": cpu, "secondary": extras,
+        "hbm_GBps_alg": achieved, "roofline": roofline, "cpu_baseline": cpu, "capi_end_to_end": capi,
+        "secondary": extras,
     }
     print(json.dumps(line, default=float), flush=True)
 

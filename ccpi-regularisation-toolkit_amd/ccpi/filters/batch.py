@@ -54,6 +54,8 @@ def _lib():
     lib.cilreg_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_float),
                                        ctypes.c_void_p]
     lib.cilreg_server_poll.restype = ctypes.c_int
+    lib.cilreg_server_status.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    lib.cilreg_server_status.restype = ctypes.c_int
     lib.cilreg_server_destroy.argtypes = [ctypes.c_void_p]
     lib.cilreg_server_destroy.restype = None
     return lib
@@ -184,28 +186,31 @@ class Server:
             raise RuntimeError(f"cilreg_server_step failed: {last_error()}")
         return a.value, p.value
 
-    def poll(self, ticket, out=None):
+    def status(self, ticket):
+        """'pending', 'running', 'done' or 'failed'."""
+        return {0: "pending", 1: "running", 2: "done"}.get(self._lib.cilreg_server_status(self._h, ticket), "failed")
+
+    def poll(self, ticket, out=None, discard=False):
         """(out, infovector) once the job is done (its memory is then released), else None.
 
         out: a float32 CUDA tensor of the job's shape to receive the result
-        (allocated when None)."""
+        (allocated when None); discard=True releases the job without copying
+        the result (out is then None)."""
         import torch
         keep = self._keep.get(ticket)
         if keep is None:
             raise KeyError(ticket)
-        info = (ctypes.c_float * 2)()
-        if out is None:
-            if len(keep) < 3:   # one result buffer per ticket, reused across polls
-                keep = keep + (torch.empty(keep[1], dtype=torch.float32, device=torch.device("cuda", self.device)),)
-                self._keep[ticket] = keep
-            out = keep[2]
-        rc = self._lib.cilreg_server_poll(self._h, ticket, info, ctypes.c_void_p(out.data_ptr()))
-        if rc < 0:
-            self._keep.pop(ticket, None)
-            raise RuntimeError(f"job {ticket} failed: {last_error()}")
-        if rc == 0:
+        st = self._lib.cilreg_server_status(self._h, ticket)
+        if st in (0, 1):
             return None
+        info = (ctypes.c_float * 2)()
+        if out is None and not discard and st == 2:
+            out = torch.empty(keep[1], dtype=torch.float32, device=torch.device("cuda", self.device))
+        rc = self._lib.cilreg_server_poll(self._h, ticket, info, ctypes.c_void_p(out.data_ptr() if out is not None
+                                                                                 else None))
         self._keep.pop(ticket, None)
+        if rc < 0:
+            raise RuntimeError(f"job {ticket} failed: {last_error()}")
         return out, np.array([info[0], info[1]], dtype=np.float32)
 
     def close(self):
@@ -224,3 +229,58 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def job_voxels(spec):
+    shape = tuple(spec["input"].shape) if spec.get("input") is not None else tuple(spec["shape"])
+    return int(np.prod(shape))
+
+
+def serve_stream(server, requests, levels=8, keep_results=False, clock=None):
+    """Replay a timed request stream through `server` in real time.
+
+    requests: list of (arrival_seconds, job dict), sorted by arrival.  Each
+    request is submitted once the wall clock passes its arrival time, all
+    admitted jobs advance `levels` iterations per server step, and finished
+    jobs are collected after every step.  Returns a dict with per-request
+    latency (completion - arrival, seconds), iterations, voxel counts, the
+    makespan and the mean number of jobs on the device per step; with
+    keep_results the outputs too (ticket order = request order)."""
+    import time
+    clock = clock or time.perf_counter
+    n = len(requests)
+    lat = [None] * n
+    iters = [0] * n
+    outs = [None] * n if keep_results else None
+    vox = [job_voxels(sp) for _, sp in requests]
+    t0 = clock()
+    nxt = 0
+    live = {}
+    steps = 0
+    occupancy = 0
+    while nxt < n or live:
+        now = clock() - t0
+        while nxt < n and requests[nxt][0] <= now:
+            live[server.submit(requests[nxt][1])] = nxt
+            nxt += 1
+        if not live:
+            time.sleep(max(0.0, min(requests[nxt][0] - now, 0.01)))
+            continue
+        active, _ = server.step(levels)
+        steps += 1
+        occupancy += active
+        for t in list(live):
+            r = server.poll(t, discard=not keep_results)
+            if r is None:
+                continue
+            i = live.pop(t)
+            lat[i] = clock() - t0 - requests[i][0]
+            iters[i] = int(r[1][0])
+            if keep_results:
+                outs[i] = r
+    makespan = clock() - t0 - (requests[0][0] if n else 0.0)
+    res = {"latency_s": lat, "iterations": iters, "voxels": vox, "makespan_s": makespan,
+           "steps": steps, "mean_active_per_step": occupancy / steps if steps else 0.0}
+    if keep_results:
+        res["results"] = outs
+    return res

@@ -23,15 +23,15 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 class CilregOpts(ctypes.Structure):
     _fields_ = [("check_every", ctypes.c_int), ("dual_precision", ctypes.c_int), ("tile_y", ctypes.c_int),
                 ("zchunk", ctypes.c_int), ("profile", ctypes.c_int), ("wide_index", ctypes.c_int),
-                ("stream_launches", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("stream_launches", ctypes.c_int), ("temporal_depth", ctypes.c_int)]
 
 
 def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0, profile=False, wide_index=False,
-              stream_launches=False):
+              stream_launches=False, depth=0):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
     return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk), int(bool(profile)),
-                      int(bool(wide_index)), int(bool(stream_launches)))
+                      int(bool(wide_index)), int(bool(stream_launches)), int(depth))
 
 
 def last_profile():
@@ -51,6 +51,18 @@ def release_cache():
     lib = require_gpu_lib()
     lib.cilreg_release_cache.restype = ctypes.c_int
     return lib.cilreg_release_cache()
+
+
+def stream_bench(nbytes=1 << 30, reps=20):
+    """Achievable HBM GB/s of the current device from the library's streaming
+    kernels: {'copy', 'read', 'write', 'sum2'} (sum2: 2 reads + 1 write)."""
+    lib = require_gpu_lib()
+    lib.cilreg_stream_bench.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.cilreg_stream_bench.restype = ctypes.c_int
+    out = (ctypes.c_double * 4)()
+    if lib.cilreg_stream_bench(int(nbytes), int(reps), out) != 0:
+        raise RuntimeError(f"cilreg_stream_bench failed: {last_error()}")
+    return dict(zip(("copy", "read", "write", "sum2"), list(out)))
 
 
 _vp = ctypes.c_void_p

@@ -336,6 +336,7 @@ struct EJob {
     int start_level = 0;
     const float* result = nullptr;   // final buffer once done
     float info[2] = {0, 0};
+    char err[160] = {};   // why the job failed
 };
 
 // group key: method, dimensionality, projection, index width
@@ -355,6 +356,7 @@ public:
     virtual int submit(const cilreg_job& j, long long* ticket) = 0;
     virtual int step(int levels, int* n_active, int* n_pending) = 0;
     virtual int poll(long long ticket, float* info, float* d_dst) = 0;
+    virtual int status(long long ticket) const = 0;
     virtual int run_batch(cilreg_job* jobs, int n) = 0;
 };
 
@@ -412,7 +414,11 @@ public:
         auto it = all_.find(ticket);
         if (it == all_.end()) return -1;
         EJob& j = *it->second;
-        if (j.state == J_FAILED) { all_.erase(it); return -1; }
+        if (j.state == J_FAILED) {
+            set_error("%s", j.err[0] ? j.err : "job failed");
+            all_.erase(it);
+            return -1;
+        }
         if (j.state != J_DONE) return 0;
         if (info) { info[0] = j.info[0]; info[1] = j.info[1]; }
         if (d_dst) {
@@ -422,6 +428,12 @@ public:
         used_ -= (double)j.bytes;
         all_.erase(it);
         return 1;
+    }
+
+    int status(long long ticket) const override {
+        auto it = all_.find(ticket);
+        if (it == all_.end() || it->second->state == J_FAILED) return -1;
+        return it->second->state;   // J_PENDING 0, J_ACTIVE 1, J_DONE 2
     }
 
     int run_batch(cilreg_job* jobs, int n) override {
@@ -544,12 +556,11 @@ private:
         return 0;
     }
 
-    // allocate and initialise one job's device state
+    // initialise one job's device state (its arena is allocated)
     int alloc_job(EJob& j) {
-        Arena& ar = *j.ar;
+        Arena& ar = *j.ar;   // allocated by the caller
         const size_t nT = j.n * sizeof(T), nS = j.n * sizeof(S);
         const int d = j.dim, nq = d == 3 ? 6 : 3;
-        if (ar.allocate() != 0) return -1;
         for (int i = 0; i < j.nfld; ++i) j.fld[i] = ar.at(j.off_fld[i]);
         j.ctrl = (Ctrl*)ar.at(j.off_ctrl);
         j.partials = (double*)ar.at(j.off_part);
@@ -593,12 +604,26 @@ private:
             if ((double)j->bytes > budget_) {
                 pending_.pop_front();
                 j->state = J_FAILED;
-                set_error("job %lld needs %zu bytes, above the budget", j->ticket, j->bytes);
+                snprintf(j->err, sizeof j->err, "job %lld needs %zu bytes, above the HBM budget of %.0f", j->ticket,
+                         j->bytes, budget_);
                 continue;
             }
             if (used_ + (double)j->bytes > budget_) break;
+            if (j->ar->allocate() != 0) {
+                // the device is fuller than the budget (other users, cached
+                // blocks): wait for running jobs to free memory
+                if (!active_.empty()) break;
+                pending_.pop_front();
+                j->state = J_FAILED;
+                snprintf(j->err, sizeof j->err, "%s", last_error());
+                return -1;
+            }
             pending_.pop_front();
-            if (alloc_job(*j) != 0) { j->state = J_FAILED; return -1; }
+            if (alloc_job(*j) != 0) {
+                j->state = J_FAILED;
+                snprintf(j->err, sizeof j->err, "%s", last_error());
+                return -1;
+            }
             used_ += (double)j->bytes;
             j->start_level = level_;
             j->state = J_ACTIVE;
@@ -948,6 +973,12 @@ int cilreg_server_poll(cilreg_server* srv, long long ticket, float* infovector, 
     std::lock_guard<std::mutex> lk(srv->mu);
     CILREG_CK(hipSetDevice(srv->dev));
     return srv->eng->poll(ticket, infovector, d_dst);
+}
+
+int cilreg_server_status(cilreg_server* srv, long long ticket) {
+    if (!srv) return -1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    return srv->eng->status(ticket);
 }
 
 void cilreg_server_destroy(cilreg_server* srv) {

@@ -43,6 +43,34 @@ static void launch_fgp(const FgpArgs<T, S>& a, bool iso, bool small, hipStream_t
 // Device-resident FGP solve.  d_out receives D of the final iteration (left
 // untouched when iter == 0); with epsil != 0 its prior contents seed the first
 // residual (FGP_TV_core.c:66,123).
+template <int TY, typename T, typename S>
+static const void* fgp2_kernel_ptr(bool iso, bool small) {
+    if (small)
+        return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint32_t>
+                   : (const void*)fgp2_iter_kernel<TY, T, S, false, uint32_t>;
+    return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint64_t>
+               : (const void*)fgp2_iter_kernel<TY, T, S, false, uint64_t>;
+}
+
+template <int TY, typename T, typename S>
+static void launch_fgp2(const Fgp2Args<T, S>& a, bool iso, bool small, hipStream_t s) {
+    dim3 block(TX, TY), grid(a.g.nblocks);
+    if (small) {
+        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint32_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint32_t>), grid, block, 0, s, a);
+    } else {
+        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint64_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint64_t>), grid, block, 0, s, a);
+    }
+}
+
+// Device-resident FGP solve.  d_out receives D of the final iteration (left
+// untouched when iter == 0); with epsil != 0 its prior contents seed the first
+// residual (FGP_TV_core.c:66,123).  In 3D, iterations run two per pass
+// (fgp2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
+// 2D iteration, runs on the one-iteration kernel.  P_j lives in buffer
+// j mod nbuf (nbuf = 4 with two-iteration passes, which write P_{k+1} and
+// P_{k+2} while reading P_k and P_{k-1}; 3 otherwise).
 template <typename T, typename S>
 int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, int nonneg, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re) {
@@ -55,9 +83,14 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
+    else if (width == 2)
+        // the pass evaluates about three planes below and above its chunk
+        g = ty == 8 ? plan_geom(fgp2_kernel_ptr<8, T, S>(iso, small), TX * 8, nx, ny, nz, 8, opts.zchunk, 3.0)
+                    : plan_geom(fgp2_kernel_ptr<4, T, S>(iso, small), TX * 4, nx, ny, nz, 4, opts.zchunk, 3.0);
     else if (ty == 8)
         g = plan_geom(small ? fgp_kernel_ptr<3, 8, T, S, uint32_t>(iso) : fgp_kernel_ptr<3, 8, T, S, uint64_t>(iso),
                       TX * 8, nx, ny, nz, 8, opts.zchunk);
@@ -65,48 +98,92 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
         g = plan_geom(small ? fgp_kernel_ptr<3, 4, T, S, uint32_t>(iso) : fgp_kernel_ptr<3, 4, T, S, uint64_t>(iso),
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
 
+    const int nbuf = width == 2 ? 4 : 3;
     Arena ar;
-    size_t oP[3][3], oD[2], oCtrl, oPart;
-    for (int b = 0; b < 3; ++b)
+    size_t oP[4][3], oD[2], oCtrl, oPart;
+    for (int b = 0; b < nbuf; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
     for (int b = 0; b < 2; ++b) oD[b] = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
-    oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
+    oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
-    // P_0 = P_{-1} = 0 (buffers 0 and 2); buffer 1 receives P_1.
+    // P_0 = P_{-1} = 0
     for (int c = 0; c < dim; ++c) {
         launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
-        launch_zero(ar.at(oP[2][c]), n * sizeof(S), s);
+        launch_zero(ar.at(oP[nbuf - 1][c]), n * sizeof(S), s);
     }
     const bool use_tol = epsil != (T)0;
     T* Dbuf[2] = {(T*)ar.at(oD[0]), (T*)ar.at(oD[1])};
     if (use_tol) launch_copy(Dbuf[1], d_out, n * sizeof(T), s);   // D_{-1} = caller's Output
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
-    double* partials = (double*)ar.at(oPart);
+    double* partials0 = (double*)ar.at(oPart);
+    double* partials1 = partials0 + 2 * (size_t)g.nblocks;
     const int ce = opts.check_every;
+    auto Pbuf = [&](int j, int c) -> S* { return (S*)ar.at(oP[((j % nbuf) + nbuf) % nbuf][c < dim ? c : 0]); };
+    // D_k is needed when iteration k is the last, is checked, or precedes a check
+    auto needs_d = [&](int k) {
+        return k == iter - 1 || (use_tol && (k % ce == 0 || (k + 1) % ce == 0));
+    };
+    const T stepc = (T)1 / ((dim == 3 ? (T)12 : (T)8) * lambda);
 
-    // t_k sequence in the working precision (FGP_TV_core.c:138,159)
-    T tk = 1, mult = 0;
-    auto step = [&](int k, bool check) {
+    // t_k sequence in the working precision (FGP_TV_core.c:138,159): mult[k]
+    // multiplies (P_k - P_{k-1}) in iteration k
+    std::vector<T> mult(iter + 1);
+    {
+        T tk = 1, m = 0;
+        for (int k = 0; k <= iter; ++k) {
+            mult[k] = m;
+            const T tkp1 = ((T)1 + dsqrt_host(((T)1 + (T)4 * tk * tk))) * (T)0.5;
+            m = (tk - (T)1) / tkp1;
+            tk = tkp1;
+        }
+    }
+    auto step = [&](int k, int nk, bool c0, bool c1) {
+        if (nk == 2) {
+            Fgp2Args<T, S> a;
+            a.f = d_f;
+            for (int c = 0; c < 3; ++c) {
+                a.Pm[c] = Pbuf(k - 1, c);
+                a.P[c] = Pbuf(k, c);
+                a.Pn[c] = Pbuf(k + 1, c);
+                a.Pnn[c] = Pbuf(k + 2, c);
+            }
+            a.D0 = Dbuf[k & 1];
+            a.D1 = Dbuf[(k + 1) & 1];
+            a.Dprev = Dbuf[(k + 1) & 1];
+            a.lambda = lambda;
+            a.step = stepc;
+            a.mult0 = mult[k];
+            a.mult1 = mult[k + 1];
+            a.write_d0 = needs_d(k) || c0;
+            a.write_d1 = needs_d(k + 1) || c1;
+            a.check0 = c0;
+            a.check1 = c1;
+            a.nonneg = nonneg != 0;
+            a.partials0 = partials0;
+            a.partials1 = partials1;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (ty == 8) launch_fgp2<8>(a, iso, small, s);
+            else launch_fgp2<4>(a, iso, small, s);
+            return;
+        }
         FgpArgs<T, S> a;
         a.f = d_f;
-        const int cur = k % 3, prv = (k + 2) % 3, nxt = (k + 1) % 3;
         for (int c = 0; c < 3; ++c) {
-            const int cc = c < dim ? c : 0;
-            a.P[c] = (const S*)ar.at(oP[cur][cc]);
-            a.Pp[c] = (const S*)ar.at(oP[prv][cc]);
-            a.Pn[c] = (S*)ar.at(oP[nxt][cc]);
+            a.P[c] = Pbuf(k, c);
+            a.Pp[c] = Pbuf(k - 1, c);
+            a.Pn[c] = Pbuf(k + 1, c);
         }
         a.Dout = Dbuf[k & 1];
         a.Dprev = Dbuf[(k + 1) & 1];
         a.lambda = lambda;
-        a.step = (T)1 / ((dim == 3 ? (T)12 : (T)8) * lambda);
-        a.mult = mult;
-        const bool next_checks = use_tol && ((k + 1) % ce == 0);
-        a.write_d = (k == iter - 1) || check || next_checks;
-        a.check = check;
+        a.step = stepc;
+        a.mult = mult[k];
+        a.write_d = needs_d(k) || c0;
+        a.check = c0;
         a.nonneg = nonneg != 0;
-        a.partials = partials;
+        a.partials = partials0;
         a.ctrl = ctrl;
         a.g = g;
         if (dim == 3) {
@@ -115,14 +192,10 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
         } else {
             launch_fgp<2, 4>(a, iso, small, s);
         }
-        // advance t for the next iteration's R (the reference updates R at the
-        // end of iteration k with t_k and t_{k+1})
-        const T tkp1 = ((T)1 + dsqrt_host(((T)1 + (T)4 * tk * tk))) * (T)0.5;
-        mult = (tk - (T)1) / tkp1;
-        tk = tkp1;
     };
     int done = 0;
-    if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
+    if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials0, partials1, g.nblocks, s, step, &done, re) != 0)
+        return -1;
     *iters_done = done;
     // the last D written: iteration `done` on a stop, else iter - 1
     const int last = (done < iter) ? done : iter - 1;
@@ -130,7 +203,6 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
-
 template int fgp_solve<float, float>(const float*, float*, float, int, float, int, int, int, int, int, const Opts&,
                                      hipStream_t, int*, float*);
 template int fgp_solve<float, bf16_t>(const float*, float*, float, int, float, int, int, int, int, int, const Opts&,

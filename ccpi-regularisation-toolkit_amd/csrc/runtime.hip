@@ -268,13 +268,14 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.profile = o->profile;
         r.wide_index = o->wide_index;
         r.stream_launches = o->stream_launches;
+        r.depth = o->temporal_depth;
     }
     return r;
 }
 
-int plan_zchunk(long long tiles, int nz, long long resident) {
-    // a chunk's prologue costs about one plane of dual work
-    const double prologue = 1.0;
+int plan_zchunk(long long tiles, int nz, long long resident, double prologue) {
+    // prologue: the extra planes a chunk evaluates before its first output
+    // plane (about one for the one-iteration kernels)
     int best = 1;
     double best_cost = 1e300;
     for (int c = 1; c <= nz; ++c) {

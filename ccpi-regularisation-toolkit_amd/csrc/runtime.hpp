@@ -130,10 +130,11 @@ struct Opts {
     int profile = 0;       // time each iteration with HIP events
     int wide_index = 0;    // force 64-bit element indexing
     int stream_launches = 0;  // batch: launch directly instead of one hipGraph
+    int depth = 0;         // iterations per pass where supported (FGP 3D: 2 by default; 1 = off)
 };
 
 // Per-iteration kernel timing of the last profiled solve on this thread.
-struct ProfileStats { double mean_ms = 0, min_ms = 0; int count = 0; };
+struct ProfileStats { double mean_ms = 0, min_ms = 0; int count = 0, passes = 0; };
 ProfileStats& thread_profile();
 Opts resolve_opts(const cilreg_opts* o);
 
@@ -144,7 +145,7 @@ void auto_tiling(int nx, int ny, int nz, int* ty, int* zc);
 // Number of z chunks that fills whole rounds of resident workgroups:
 // minimises rounds * (planes per chunk + prologue cost) for `tiles` xy tiles
 // and `resident` concurrently resident workgroups on the device.
-int plan_zchunk(long long tiles, int nz, long long resident);
+int plan_zchunk(long long tiles, int nz, long long resident, double prologue = 1.0);
 
 // Workgroups of `kernel` (with `threads` threads) resident on the whole
 // current device at once.
@@ -161,7 +162,7 @@ long long resident_blocks(K kernel, int threads) {
 // Geometry for a marching kernel: explicit zchunk if given, else planned
 // from the kernel's residency.
 template <class K>
-Geom plan_geom(K kernel, int threads, int nx, int ny, int nz, int ty, int zc_hint) {
+Geom plan_geom(K kernel, int threads, int nx, int ny, int nz, int ty, int zc_hint, double prologue = 1.0) {
     if (zc_hint > 0 || nz <= 1) return make_geom(nx, ny, nz, ty, zc_hint > 0 ? zc_hint : 1);
     const long long tiles = (long long)((nx + TX - 1) / TX) * ((ny + ty - 1) / ty);
     const long long res = resident_blocks(kernel, threads);
@@ -170,18 +171,23 @@ Geom plan_geom(K kernel, int threads, int nx, int ny, int nz, int ty, int zc_hin
         auto_tiling(nx, ny, nz, &t, &zc);
         return make_geom(nx, ny, nz, ty, zc);
     }
-    const int chunks = plan_zchunk(tiles, nz, res);
+    const int chunks = plan_zchunk(tiles, nz, res, prologue);
     return make_geom(nx, ny, nz, ty, (nz + chunks - 1) / chunks);
 }
 
-// Drives `iter` iterations on stream s.  step(k, check) launches iteration k
-// (check: whether this iteration evaluates the stopping rule); the driver then
-// launches the finaliser, polls the stop flag every opts.poll iterations
-// without draining the queue, and on return has synchronised the stream and
-// filled *iters_done and *re.
+// Drives `iter` iterations on stream s in passes.  step(k, nk, check0,
+// check1) launches the pass that advances iterations k .. k+nk-1 (nk is 1 or,
+// when the caller's pass width is 2 and two iterations remain, 2); check0 /
+// check1 say whether iteration k / k+1 evaluates the stopping rule, and the
+// pass leaves those residual sums in partials0 / partials1.  The driver then
+// launches the finalisers in iteration order, polls the stop flag every
+// opts.poll iterations without draining the queue, and on return has
+// synchronised the stream and filled *iters_done and *re.  With opts.profile
+// every pass is timed with HIP events; the profile reports time per
+// iteration (pass time / nk).
 template <class StepFn>
-int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, double* d_partials,
-                     int nblocks, hipStream_t s, StepFn&& step, int* iters_done, float* re) {
+int drive_passes(int iter, int width, float epsil, const Opts& opts, Ctrl* d_ctrl, double* d_partials0,
+                 double* d_partials1, int nblocks, hipStream_t s, StepFn&& step, int* iters_done, float* re) {
     PinnedCtrl pin;
     const bool use_tol = epsil != 0.0f;
     if (use_tol && pin.allocate() != 0) return -1;
@@ -192,23 +198,35 @@ int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, doub
         CILREG_CK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         CILREG_CK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     }
-    int poll_idx = 0;
+    int poll_idx = 0, since_poll = 0;
     int rc = 0;
     std::vector<hipEvent_t> pev;
+    std::vector<int> pass_k, pass_n;
     if (opts.profile) {
         pev.resize(2 * (size_t)iter, nullptr);
         for (auto& e : pev)
             if (hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); rc = -1; break; }
     }
-    for (int k = 0; k < iter && rc == 0; ++k) {
-        const bool check = use_tol && (k % opts.check_every == 0);
-        if (opts.profile) (void)hipEventRecord(pev[2 * k], s);
-        step(k, check);
-        if (opts.profile) (void)hipEventRecord(pev[2 * k + 1], s);
-        if (check) launch_finalize(d_partials, nblocks, d_ctrl, epsil, k, s);
-        if (use_tol && ((k + 1) % opts.poll == 0) && k + 1 < iter) {
+    int npass = 0;
+    for (int k = 0; k < iter && rc == 0;) {
+        const int nk = (width >= 2 && k + 1 < iter) ? 2 : 1;
+        const bool c0 = use_tol && (k % opts.check_every == 0);
+        const bool c1 = nk == 2 && use_tol && ((k + 1) % opts.check_every == 0);
+        if (opts.profile) (void)hipEventRecord(pev[2 * npass], s);
+        step(k, nk, c0, c1);
+        if (opts.profile) {
+            (void)hipEventRecord(pev[2 * npass + 1], s);
+            pass_k.push_back(k);
+            pass_n.push_back(nk);
+        }
+        ++npass;
+        if (c0) launch_finalize(d_partials0, nblocks, d_ctrl, epsil, k, s);
+        if (c1) launch_finalize(d_partials1, nblocks, d_ctrl, epsil, k + 1, s);
+        k += nk;
+        since_poll += nk;
+        if (use_tol && since_poll >= opts.poll && k < iter) {
+            since_poll = 0;
             const int slot = poll_idx & 1;
-            // pinned slot: h[slot].stopped
             if (hipMemcpyAsync(&pin.h[slot].stopped, &d_ctrl->stopped, sizeof(int), hipMemcpyDeviceToHost, s) !=
                     hipSuccess ||
                 hipEventRecord(ev[slot], s) != hipSuccess) {
@@ -239,24 +257,36 @@ int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, doub
     if (opts.profile) {
         ProfileStats& ps = thread_profile();
         ps = ProfileStats();
+        // passes that ran (a stop at iteration j leaves later passes as no-ops)
         const int ran = rc == 0 ? (c.stopped ? c.stop_iter + 1 : iter) : 0;
         double sum = 0, mn = 1e300;
-        int cnt = 0;
-        for (int k = 0; k < ran; ++k) {
+        int cnt = 0, its = 0;
+        for (size_t q = 0; q < pass_k.size(); ++q) {
+            if (pass_k[q] >= ran) break;
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, pev[2 * k], pev[2 * k + 1]) == hipSuccess) {
+            if (hipEventElapsedTime(&ms, pev[2 * q], pev[2 * q + 1]) == hipSuccess) {
                 sum += ms;
-                mn = ms < mn ? ms : mn;
+                its += pass_n[q];
+                const double per = ms / pass_n[q];
+                mn = per < mn ? per : mn;
                 ++cnt;
             }
         }
-        if (cnt) { ps.mean_ms = sum / cnt; ps.min_ms = mn; ps.count = cnt; }
-        for (auto& e : pev) if (e) (void)hipEventDestroy(e);
+        if (cnt) { ps.mean_ms = sum / its; ps.min_ms = mn; ps.count = its; ps.passes = cnt; }
+        for (auto& ev2 : pev) if (ev2) (void)hipEventDestroy(ev2);
     }
     if (rc != 0) return rc;
     *iters_done = c.stopped ? c.stop_iter : iter;
     *re = c.re;
     return 0;
+}
+
+// One iteration per pass: step(k, check) launches iteration k.
+template <class StepFn>
+int drive_iterations(int iter, float epsil, const Opts& opts, Ctrl* d_ctrl, double* d_partials,
+                     int nblocks, hipStream_t s, StepFn&& step, int* iters_done, float* re) {
+    return drive_passes(iter, 1, epsil, opts, d_ctrl, d_partials, d_partials, nblocks, s,
+                        [&](int k, int, bool c0, bool) { step(k, c0); }, iters_done, re);
 }
 
 }  // namespace cilreg

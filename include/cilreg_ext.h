@@ -19,6 +19,8 @@
 #ifndef CILREG_EXT_H
 #define CILREG_EXT_H
 
+#include <stddef.h>
+
 #ifndef CILREG_API
 #if defined(__GNUC__) || defined(__clang__)
 #define CILREG_API __attribute__((visibility("default")))
@@ -41,7 +43,7 @@ typedef struct cilreg_opts {
     int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
     int wide_index;      /* 1: force 64-bit element indexing (chosen automatically for fields >= 2 GiB) */
     int stream_launches; /* batch: 1 = launch directly instead of capturing one hipGraph */
-    int reserved;
+    int temporal_depth;  /* iterations fused per pass where supported (FGP 3D): 0 = auto (2), 1 = off */
 } cilreg_opts;
 
 /* Methods of a batch or served job. */
@@ -75,8 +77,9 @@ CILREG_API const char *cilreg_last_error(void);
 
 /* Per-iteration kernel time of the last solve on this host thread that ran
  * with opts.profile = 1: mean and minimum in milliseconds over `count`
- * iterations, measured with HIP events on the solve's stream around the
- * iteration's main kernel(s) (the stopping-rule finaliser excluded). */
+ * iterations, measured with HIP events on the solve's stream around each
+ * launch of the iteration kernel(s) (the stopping-rule finaliser excluded);
+ * a launch that advances several iterations counts its time divided by them. */
 CILREG_API int cilreg_last_profile(double *mean_ms, double *min_ms, int *count);
 
 /* Frees the device memory the library keeps cached between calls on the
@@ -110,6 +113,12 @@ CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, doubl
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts *opts, void *stream);
 
+/* Achievable HBM bandwidth of the current device, from streaming kernels
+ * over arrays of `bytes` each (>= 1 MiB), the mean of `reps` timed launches:
+ * gbps[0] copy (1 read + 1 write), gbps[1] read, gbps[2] write, gbps[3]
+ * 2 reads + 1 write.  Bytes moved / kernel time, in GB/s. */
+CILREG_API int cilreg_stream_bench(size_t bytes, int reps, double *gbps);
+
 /* Runs every job of the batch to completion with grouped kernels: one launch
  * per iteration level and method group covers all volumes of that group, so
  * many small or uneven volumes fill the device together.  Unless
@@ -136,6 +145,9 @@ CILREG_API int cilreg_server_step(cilreg_server *srv, int levels, int *n_active,
 /* 1 when done (infovector filled; the output copied to d_dst if not NULL, then
  * the job's memory is released), 0 when pending or running, -1 if unknown. */
 CILREG_API int cilreg_server_poll(cilreg_server *srv, long long ticket, float *infovector, float *d_dst);
+/* 0 waiting for admission, 1 running, 2 done (collect with cilreg_server_poll),
+ * -1 failed (e.g. larger than the budget; poll reports the error) or unknown. */
+CILREG_API int cilreg_server_status(cilreg_server *srv, long long ticket);
 CILREG_API void cilreg_server_destroy(cilreg_server *srv);
 
 #ifdef __cplusplus

@@ -1,0 +1,72 @@
+"""Two-iteration FGP passes (temporal blocking) against one-iteration passes.
+
+The 3D FGP solver fuses iterations k and k+1 into one pass by default; the
+pass evaluates iteration k on a ring around its tile and every plane through
+a four-stage z pipeline.  Each value is computed by the same expressions in
+the same order as the one-iteration kernel, so the results and the
+infovector must be bit-identical to depth 1 (itself pinned to the oracle by
+test_gpu_parity.py) for every tiling, chunking (down to one plane per
+workgroup), boundary, TV type, nonnegativity, storage precision, odd and even
+iteration counts, and tolerance stops at every check cadence.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(20, 36, 70), (9, 13, 130), (2, 17, 65), (33, 64, 64), (5, 70, 3)]
+
+
+def _vol(shape, seed):
+    from ccpi.filters import batch as B
+    return B.phantom(shape, 6, 0.05, seed)
+
+
+def _pair(x, **kw):
+    from ccpi.filters import device as D
+    a = D.fgp(x, depth=1, **kw)
+    b = D.fgp(x, depth=2, **kw)
+    return a, b
+
+
+def _same(a, b, tag):
+    (ya, ia), (yb, ib) = a, b
+    assert np.array_equal(ia, ib), f"{tag}: infovector {ia} vs {ib}"
+    assert bool((ya == yb).all()), f"{tag}: {int((ya != yb).sum())} elements differ"
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("zchunk", [0, 1, 2, 3, 7])
+def test_depth2_matches_depth1_tilings(shape, zchunk):
+    x = _vol(shape, 11)
+    for tile_y in (4, 8):
+        for iters in (1, 2, 7, 10):
+            a, b = _pair(x, lam=0.03, iters=iters, zchunk=zchunk, tile_y=tile_y, nonneg=1)
+            _same(a, b, f"{shape} zchunk={zchunk} tile_y={tile_y} iters={iters}")
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["iso", "aniso"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_depth2_matches_depth1_methods(method, precision):
+    x = _vol((24, 40, 72), 12)
+    for nonneg in (0, 1):
+        a, b = _pair(x, lam=0.02, iters=25, method=method, nonneg=nonneg, precision=precision)
+        _same(a, b, f"method={method} nonneg={nonneg} {precision}")
+
+
+@pytest.mark.parametrize("check_every", [1, 2, 3, 5])
+def test_depth2_matches_depth1_tolerance_stops(check_every):
+    x = _vol((16, 48, 80), 13)
+    stopped = False
+    for tol in (1e-2, 3e-3, 1e-4):
+        for zchunk in (0, 4):
+            a, b = _pair(x, lam=0.03, iters=120, tol=tol, check_every=check_every, zchunk=zchunk)
+            _same(a, b, f"tol={tol} check_every={check_every} zchunk={zchunk}")
+            stopped |= int(a[1][0]) < 120
+    assert stopped
+
+
+def test_depth2_wide_index():
+    x = _vol((12, 30, 66), 14)
+    a, b = _pair(x, lam=0.03, iters=9, wide_index=True)
+    _same(a, b, "wide index")
