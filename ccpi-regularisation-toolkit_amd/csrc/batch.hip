@@ -52,7 +52,9 @@ template <typename T> struct RofJob {
 template <typename T, typename S> struct FgpJob {
     JobHdr h;
     const T* f;
-    S* P[3][3];
+    S* P[4][3];         // P_j in set j mod nbuf
+    double* partials1;  // residual sums of the second iteration of a pass
+    int nbuf;           // 4 for two-iteration passes (3D), 3 otherwise
     T* D[2];
     const T* mult;      // FISTA multipliers by iteration index
     T lambda, step;
@@ -74,12 +76,15 @@ template <typename T, typename S> struct TgvJob {
     S* V[2][3];
     S* P[2][3];
     S* Q[2][6];
+    T* Ualt;            // 3D: u_bar after the first iteration of a checked pass
+    double* partials1;  // 3D: residual sums of the second iteration of a pass
     T lambda, tau, alpha1, alpha0;
 };
 
 struct FinJob {
     Ctrl* ctrl;
     const double* partials;
+    const double* partials1;   // second iteration of a two-iteration pass
     int nblocks, start_level, iters, check_every;
     float epsil;
 };
@@ -101,13 +106,40 @@ template <class J>
 __device__ inline int group_prologue(const J* jobs, int njobs, int total_blocks, int level, const J*& job,
                                      int& local) {
     const int L = xcd_logical_block((int)blockIdx.x, total_blocks);
-    const int j = find_job(jobs, njobs, L);
+    // the job index is uniform over the workgroup; saying so lets the
+    // descriptor be read with scalar loads into scalar registers
+    const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, L));
     job = &jobs[j];
     const int k = level - job->h.start_level;
     if (k < 0 || k >= job->h.iters) return -1;
     if (job->h.ctrl->stopped) return -1;
     local = L - job->h.block_offset;
     return k;
+}
+
+// A pointer read from a job descriptor, made explicitly uniform: it then
+// lives in scalar registers like a kernel argument instead of a vector
+// register pair per pointer (the group kernels would otherwise need about 30
+// more vector registers than the single-volume kernels, and spill).
+template <class P>
+__device__ inline P* uni(P* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (P*)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline float unf(float x) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x)));
+}
+__device__ inline int uni_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline Geom uni_geom(const Geom& g0) {
+    Geom g;
+    g.nx = uni_i(g0.nx); g.ny = uni_i(g0.ny); g.nz = uni_i(g0.nz);
+    g.tiles_x = uni_i(g0.tiles_x); g.tiles_y = uni_i(g0.tiles_y);
+    g.zc = uni_i(g0.zc); g.nzc = uni_i(g0.nzc); g.nblocks = uni_i(g0.nblocks);
+    g.sxy = (long long)g.nx * g.ny;
+    return g;
 }
 
 __device__ inline bool job_checks(const JobHdr& h, int k) { return h.check_every > 0 && k % h.check_every == 0; }
@@ -119,15 +151,15 @@ __global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* job
     const int k = group_prologue(jobs, njobs, total, level, j, local);
     if (k < 0) return;
     RofArgs<T> a;
-    a.f = j->f;
-    a.U = j->U[k & 1];
-    a.Uout = j->U[(k + 1) & 1];
-    a.lambda = j->lambda;
-    a.tau = j->tau;
+    a.f = uni(j->f);
+    a.U = uni(j->U[k & 1]);
+    a.Uout = uni(j->U[(k + 1) & 1]);
+    a.lambda = unf(j->lambda);
+    a.tau = unf(j->tau);
     a.check = job_checks(j->h, k);
-    a.partials = j->h.partials;
-    a.ctrl = j->h.ctrl;
-    a.g = j->h.g;
+    a.partials = uni(j->h.partials);
+    a.ctrl = uni(j->h.ctrl);
+    a.g = uni_geom(j->h.g);
     rof_body<DIM, TY, T, I>(a, tile_of_logical(a.g, TY, local));
 }
 
@@ -139,27 +171,92 @@ __global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* 
     const int k = group_prologue(jobs, njobs, total, level, j, local);
     if (k < 0) return;
     FgpArgs<T, S> a;
-    const int cur = k % 3, prv = (k + 2) % 3, nxt = (k + 1) % 3;
+    const int nb = j->nbuf;
+    const int cur = k % nb, prv = (k + nb - 1) % nb, nxt = (k + 1) % nb;
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a.P[c] = j->P[cur][c];
-        a.Pp[c] = j->P[prv][c];
-        a.Pn[c] = j->P[nxt][c];
+        a.P[c] = uni(j->P[cur][c]);
+        a.Pp[c] = uni(j->P[prv][c]);
+        a.Pn[c] = uni(j->P[nxt][c]);
     }
     const bool check = job_checks(j->h, k);
-    a.f = j->f;
-    a.Dout = j->D[k & 1];
-    a.Dprev = j->D[(k + 1) & 1];
-    a.lambda = j->lambda;
-    a.step = j->step;
-    a.mult = j->mult[k];
+    a.f = uni(j->f);
+    a.Dout = uni(j->D[k & 1]);
+    a.Dprev = uni(j->D[(k + 1) & 1]);
+    a.lambda = unf(j->lambda);
+    a.step = unf(j->step);
+    a.mult = unf(j->mult[k]);
     a.write_d = (k == j->h.iters - 1) || check || (j->h.check_every > 0 && (k + 1) % j->h.check_every == 0);
     a.check = check;
     a.nonneg = j->nonneg;
-    a.partials = j->h.partials;
-    a.ctrl = j->h.ctrl;
-    a.g = j->h.g;
+    a.partials = uni(j->h.partials);
+    a.ctrl = uni(j->h.ctrl);
+    a.g = uni_geom(j->h.g);
     fgp_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
+}
+
+// 3D FGP groups advance two levels per launch: a job at an even iteration
+// index k runs the two-iteration pass, or at its last (odd-count) iteration
+// the one-iteration body.  Jobs start at even levels, so k is always even.
+template <int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total,
+                                                             int level) {
+    const FgpJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    const int ce = j->h.check_every, iters = j->h.iters;
+    auto needs_d = [&](int q) { return q == iters - 1 || (ce > 0 && (q % ce == 0 || (q + 1) % ce == 0)); };
+    const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
+    if (k + 1 < iters) {
+        Fgp2Args<T, S> a;
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.Pm[c] = uni(j->P[(k + 3) & 3][c]);
+            a.P[c] = uni(j->P[k & 3][c]);
+            a.Pn[c] = uni(j->P[(k + 1) & 3][c]);
+            a.Pnn[c] = uni(j->P[(k + 2) & 3][c]);
+        }
+        a.f = uni(j->f);
+        a.D0 = uni(j->D[k & 1]);
+        a.D1 = uni(j->D[(k + 1) & 1]);
+        a.Dprev = uni(j->D[(k + 1) & 1]);
+        a.lambda = unf(j->lambda);
+        a.step = unf(j->step);
+        a.mult0 = unf(j->mult[k]);
+        a.mult1 = unf(j->mult[k + 1]);
+        a.check0 = job_checks(j->h, k);
+        a.check1 = job_checks(j->h, k + 1);
+        a.write_d0 = needs_d(k);
+        a.write_d1 = needs_d(k + 1);
+        a.nonneg = j->nonneg;
+        a.partials0 = uni(j->h.partials);
+        a.partials1 = uni(j->partials1);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        fgp2_body<TY, T, S, ISO, I>(a, tc);
+    } else {
+        FgpArgs<T, S> a;
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.P[c] = uni(j->P[k & 3][c]);
+            a.Pp[c] = uni(j->P[(k + 3) & 3][c]);
+            a.Pn[c] = uni(j->P[(k + 1) & 3][c]);
+        }
+        a.f = uni(j->f);
+        a.Dout = uni(j->D[k & 1]);
+        a.Dprev = uni(j->D[(k + 1) & 1]);
+        a.lambda = unf(j->lambda);
+        a.step = unf(j->step);
+        a.mult = unf(j->mult[k]);
+        a.check = job_checks(j->h, k);
+        a.write_d = needs_d(k);
+        a.nonneg = j->nonneg;
+        a.partials = uni(j->h.partials);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        fgp_body<3, TY, T, S, ISO, I>(a, tc);
+    }
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
@@ -170,24 +267,24 @@ __global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jo
     if (k < 0) return;
     PdArgs<T, S> a;
     const int cur = k & 1, nxt = cur ^ 1;
-    a.f = j->f;
-    a.U = j->U[cur];
-    a.Uout = j->U[nxt];
+    a.f = uni(j->f);
+    a.U = uni(j->U[cur]);
+    a.Uout = uni(j->U[nxt]);
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a.P[c] = j->P[cur][c];
-        a.Pout[c] = j->P[nxt][c];
+        a.P[c] = uni(j->P[cur][c]);
+        a.Pout[c] = uni(j->P[nxt][c]);
     }
-    a.Ubrk = j->Ubrk;
-    a.sigma = j->sigma;
-    a.tau = j->tau;
-    a.lt = j->lt;
+    a.Ubrk = uni(j->Ubrk);
+    a.sigma = unf(j->sigma);
+    a.tau = unf(j->tau);
+    a.lt = unf(j->lt);
     a.theta = 1;
     a.nonneg = j->nonneg;
     a.check = job_checks(j->h, k);
-    a.partials = j->h.partials;
-    a.ctrl = j->h.ctrl;
-    a.g = j->h.g;
+    a.partials = uni(j->h.partials);
+    a.ctrl = uni(j->h.ctrl);
+    a.g = uni_geom(j->h.g);
     pd_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
 }
 
@@ -200,38 +297,112 @@ __global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* 
     if (k < 0) return;
     TgvArgs<T, S> a;
     const int cur = k & 1, nxt = cur ^ 1;
-    a.f = j->f;
-    a.U = j->U[cur];
-    a.Uout = j->U[nxt];
+    a.f = uni(j->f);
+    a.U = uni(j->U[cur]);
+    a.Uout = uni(j->U[nxt]);
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a.V[c] = j->V[cur][c];
-        a.Vout[c] = j->V[nxt][c];
-        a.P[c] = j->P[cur][c];
-        a.Pout[c] = j->P[nxt][c];
+        a.V[c] = uni(j->V[cur][c]);
+        a.Vout[c] = uni(j->V[nxt][c]);
+        a.P[c] = uni(j->P[cur][c]);
+        a.Pout[c] = uni(j->P[nxt][c]);
     }
     #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        a.Q[c] = j->Q[cur][c];
-        a.Qout[c] = j->Q[nxt][c];
+        a.Q[c] = uni(j->Q[cur][c]);
+        a.Qout[c] = uni(j->Q[nxt][c]);
     }
-    a.lambda = j->lambda;
-    a.tau = j->tau;
-    a.sigma = j->tau;
-    a.alpha1 = j->alpha1;
-    a.alpha0 = j->alpha0;
+    a.lambda = unf(j->lambda);
+    a.tau = unf(j->tau);
+    a.sigma = unf(j->tau);
+    a.alpha1 = unf(j->alpha1);
+    a.alpha0 = unf(j->alpha0);
     a.check = job_checks(j->h, k);
-    a.partials = j->h.partials;
-    a.ctrl = j->h.ctrl;
-    a.g = j->h.g;
+    a.partials = uni(j->h.partials);
+    a.ctrl = uni(j->h.ctrl);
+    a.g = uni_geom(j->h.g);
     tgv_body<DIM, TY, T, S, I>(a, tile_of_logical(a.g, TY, local));
 }
 
-__global__ __launch_bounds__(256) void group_finalize_kernel(const FinJob* jobs, int level) {
+// 3D TGV groups advance two levels per launch: pass q = k / 2 of a job reads
+// state set q & 1 and writes the other; a job's last iteration of an odd
+// count runs the one-iteration body on the same sets.
+template <int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_group_kernel(
+    const TgvJob<T, S>* jobs, int njobs, int total, int level) {
+    const TgvJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    const int cur = (k >> 1) & 1, nxt = cur ^ 1;
+    const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
+    if (k + 1 < j->h.iters) {
+        Tgv2Args<T, S> a;
+        a.f = uni(j->f);
+        a.U = uni(j->U[cur]);
+        a.Uout = uni(j->U[nxt]);
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.V[c] = uni(j->V[cur][c]);
+            a.Vout[c] = uni(j->V[nxt][c]);
+            a.P[c] = uni(j->P[cur][c]);
+            a.Pout[c] = uni(j->P[nxt][c]);
+        }
+        #pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            a.Q[c] = uni(j->Q[cur][c]);
+            a.Qout[c] = uni(j->Q[nxt][c]);
+        }
+        a.Ualt = uni(j->Ualt);
+        a.lambda = unf(j->lambda);
+        a.tau = unf(j->tau);
+        a.sigma = unf(j->tau);
+        a.alpha1 = unf(j->alpha1);
+        a.alpha0 = unf(j->alpha0);
+        a.check0 = job_checks(j->h, k);
+        a.check1 = job_checks(j->h, k + 1);
+        a.partials0 = uni(j->h.partials);
+        a.partials1 = uni(j->partials1);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        tgv2_body<TY, T, S, I>(a, tc);
+    } else {
+        TgvArgs<T, S> a;
+        a.f = uni(j->f);
+        a.U = uni(j->U[cur]);
+        a.Uout = uni(j->U[nxt]);
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.V[c] = uni(j->V[cur][c]);
+            a.Vout[c] = uni(j->V[nxt][c]);
+            a.P[c] = uni(j->P[cur][c]);
+            a.Pout[c] = uni(j->P[nxt][c]);
+        }
+        #pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            a.Q[c] = uni(j->Q[cur][c]);
+            a.Qout[c] = uni(j->Q[nxt][c]);
+        }
+        a.lambda = unf(j->lambda);
+        a.tau = unf(j->tau);
+        a.sigma = unf(j->tau);
+        a.alpha1 = unf(j->alpha1);
+        a.alpha0 = unf(j->alpha0);
+        a.check = job_checks(j->h, k);
+        a.partials = uni(j->h.partials);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        tgv_body<3, TY, T, S, I>(a, tc);
+    }
+}
+
+// `second`: the level is the second iteration of a two-iteration pass, whose
+// residual sums are in partials1.
+__global__ __launch_bounds__(256) void group_finalize_kernel(const FinJob* jobs, int level, int second) {
     const FinJob& j = jobs[blockIdx.x];
     const int k = level - j.start_level;
     if (k < 0 || k >= j.iters || j.check_every <= 0 || k % j.check_every != 0) return;
-    finalize_body(j.partials, j.nblocks, j.ctrl, j.epsil, k);
+    finalize_body(second ? j.partials1 : j.partials, j.nblocks, j.ctrl, j.epsil, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,6 +492,7 @@ struct EJob {
     long long ticket = 0;
     int state = J_PENDING;
     int dim = 3, nz = 1, ty = 8;
+    int nbuf = 3;   // FGP dual sets: 4 with two-iteration passes (3D)
     bool wide = false;
     size_t n = 0, bytes = 0;
     std::unique_ptr<Arena> ar;   // every device buffer of the job, one allocation
@@ -386,6 +558,7 @@ public:
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
         j->ty = j->dim == 3 ? 8 : 4;
+        j->nbuf = (j->pub.method == CILREG_FGP && j->dim == 3) ? 4 : 3;
         const int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
         j->g = make_geom(pj.dimX, pj.dimY, j->nz, j->ty, zc);
         plan_job(*j);
@@ -397,6 +570,7 @@ public:
     }
 
     int step(int levels, int* n_active, int* n_pending) override {
+        levels += levels & 1;   // even, so jobs are admitted at even levels (two-level FGP passes)
         if (admit() != 0) return -1;
         if (!active_.empty() && levels > 0) {
             if (build_tables() != 0) return -1;
@@ -465,7 +639,7 @@ public:
                 (void)hipGraphDestroy(graph);
                 if (e != hipSuccess) { set_error("batch graph launch failed: %s", hipGetErrorString(e)); return -1; }
             }
-            level_ = end;
+            level_ = end + (end & 1);
             CILREG_CK(hipStreamSynchronize(s_));
             if (retire() != 0) return -1;
         }
@@ -520,16 +694,17 @@ private:
         auto res = [&](size_t b) { j.off_fld[m++] = ar.reserve(b); };
         switch (j.pub.method) {
             case CILREG_ROF: res(nT); res(nT); break;                                   // U x2
-            case CILREG_FGP: for (int i = 0; i < 3 * d; ++i) res(nS); res(nT); res(nT); break;   // P x3 sets, D x2
+            case CILREG_FGP: for (int i = 0; i < j.nbuf * d; ++i) res(nS); res(nT); res(nT); break;   // P sets, D x2
             case CILREG_PD: res(nT); res(nT); for (int i = 0; i < 2 * d; ++i) res(nS); res(nT); break;  // U x2, P x2, Ubrk
             default:                                                                       // U x2, (V, P, Q) x2
                 res(nT); res(nT);
                 for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
+                if (d == 3 && j.pub.epsil != 0.0f) res(nT);   // Ualt of two-iteration passes
                 break;
         }
         j.nfld = m;
         j.off_ctrl = ar.reserve(sizeof(Ctrl));
-        j.off_part = ar.reserve((size_t)j.g.nblocks * 2 * sizeof(double));
+        j.off_part = ar.reserve((size_t)j.g.nblocks * 4 * sizeof(double));   // two iterations' sums
         if (!j.pub.d_input) {
             j.off_f = ar.reserve(nT);
             j.off_sh = ar.reserve(sizeof(Shape) * (size_t)std::max(j.pub.phantom_shapes, 1));
@@ -578,11 +753,13 @@ private:
         switch (j.pub.method) {
             case CILREG_ROF: launch_copy(j.fld[0], j.f, nT, s_); break;
             case CILREG_FGP:
-                for (int b = 0; b < 3; b += 2)
-                    for (int c = 0; c < d; ++c) launch_zero(j.fld[b * d + c], nS, s_);
+                for (int c = 0; c < d; ++c) {   // P_0 = P_{-1} = 0
+                    launch_zero(j.fld[c], nS, s_);
+                    launch_zero(j.fld[(j.nbuf - 1) * d + c], nS, s_);
+                }
                 if (j.pub.epsil != 0.0f) {   // D_{-1} = caller's output (FGP_TV_core.c:66,123)
-                    if (j.pub.d_output) launch_copy(j.fld[3 * d + 1], j.pub.d_output, nT, s_);
-                    else launch_zero(j.fld[3 * d + 1], nT, s_);
+                    if (j.pub.d_output) launch_copy(j.fld[j.nbuf * d + 1], j.pub.d_output, nT, s_);
+                    else launch_zero(j.fld[j.nbuf * d + 1], nT, s_);
                 }
                 if (ensure_mult(j.pub.iter) != 0) return -1;
                 break;
@@ -678,10 +855,12 @@ private:
                 FgpJob<T, S> r{};
                 fill_hdr(r, j, offset);
                 r.f = j.f;
-                for (int b = 0; b < 3; ++b)
+                for (int b = 0; b < j.nbuf; ++b)
                     for (int c = 0; c < 3; ++c) r.P[b][c] = (S*)ch(b * d, c, d);
-                r.D[0] = (T*)j.fld[3 * d];
-                r.D[1] = (T*)j.fld[3 * d + 1];
+                r.D[0] = (T*)j.fld[j.nbuf * d];
+                r.D[1] = (T*)j.fld[j.nbuf * d + 1];
+                r.nbuf = j.nbuf;
+                r.partials1 = j.partials + 2 * (size_t)j.g.nblocks;
                 r.mult = mult_;
                 r.lambda = j.pub.lambda;
                 r.step = (T)1 / ((d == 3 ? (T)12 : (T)8) * j.pub.lambda);
@@ -721,6 +900,8 @@ private:
                     }
                     for (int c = 0; c < 6; ++c) r.Q[b][c] = (S*)ch(base + 2 * d, c, nq);
                 }
+                r.Ualt = (d == 3 && j.pub.epsil != 0.0f) ? (T*)j.fld[2 + 2 * per] : nullptr;
+                r.partials1 = j.partials + 2 * (size_t)j.g.nblocks;
                 r.lambda = j.pub.lambda;
                 r.tau = (T)pow((double)j.pub.L2, -0.5);   // TGV_core.c:54-55
                 r.alpha1 = j.pub.alpha1;
@@ -751,7 +932,8 @@ private:
                 t.ends.push_back(j->start_level + j->pub.iter);
                 off += j->g.nblocks;
                 if (j->pub.epsil != 0.0f)
-                    fins.push_back(FinJob{j->ctrl, j->partials, j->g.nblocks, j->start_level, j->pub.iter,
+                    fins.push_back(FinJob{j->ctrl, j->partials, j->partials + 2 * (size_t)j->g.nblocks, j->g.nblocks,
+                                          j->start_level, j->pub.iter,
                                           opts_.check_every, j->pub.epsil});
             }
             t.offsets.push_back(off);
@@ -792,7 +974,15 @@ private:
                 break;
             case CILREG_FGP: {
                 auto* jp = (const FgpJob<T, S>*)t.dev;
-                if (w) {
+                if constexpr (DIM == 3) {
+                    if (w) {
+                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        else hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    } else {
+                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        else hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    }
+                } else if (w) {
                     if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                     else hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                 } else {
@@ -814,8 +1004,13 @@ private:
             }
             default: {
                 auto* jp = (const TgvJob<T, S>*)t.dev;
-                if (w) hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                else hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                if constexpr (DIM == 3) {
+                    if (w) hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                } else {
+                    if (w) hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    else hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                }
                 break;
             }
         }
@@ -845,16 +1040,21 @@ private:
             const GroupKey& k = groups[gi].first;
             const Table& t = *groups[gi].second;
             hipStream_t st = side_[gi];
-            for (int lv = l0; lv < l1; ++lv) {
+            // 3D FGP and TGV groups advance two levels per launch (l0 is even)
+            const int width = ((k.method == CILREG_FGP || k.method == CILREG_TGV) && k.dim == 3) ? 2 : 1;
+            for (int lv = l0; lv < l1; lv += width) {
                 int nact = 0;
                 while (nact < t.njobs && t.ends[nact] > lv) ++nact;
                 if (nact == 0) break;
                 const int total = t.offsets[nact];
                 if (k.dim == 3) launch_group<3, 8>(k, t, nact, total, lv, st);
                 else launch_group<2, 4>(k, t, nact, total, lv, st);
-                if (t.fin_n)
-                    hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st,
-                                       (const FinJob*)fin_dev_ + t.fin_off, lv);
+                if (t.fin_n) {
+                    const FinJob* fj = (const FinJob*)fin_dev_ + t.fin_off;
+                    hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st, fj, lv, 0);
+                    if (width == 2)
+                        hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st, fj, lv + 1, 1);
+                }
             }
             CILREG_CK(hipEventRecord(events_[gi], st));
             CILREG_CK(hipStreamWaitEvent(s_, events_[gi], 0));
@@ -881,9 +1081,17 @@ private:
             const int d = j.dim;
             switch (j.pub.method) {
                 case CILREG_ROF: j.result = (const float*)j.fld[(last + 1) & 1]; break;
-                case CILREG_FGP: j.result = (const float*)j.fld[3 * d + (last & 1)]; break;
+                case CILREG_FGP: j.result = (const float*)j.fld[j.nbuf * d + (last & 1)]; break;
                 case CILREG_PD: j.result = stopped ? (const float*)j.fld[2 + 2 * d] : (const float*)j.fld[iters & 1]; break;
-                default: j.result = (const float*)j.fld[(last + 1) & 1]; break;
+                default:
+                    if (d == 3) {   // pass q = last / 2 wrote set (q + 1) & 1; Ualt on a stop at its first iteration
+                        const bool alt = stopped && (last & 1) == 0 && last + 1 < iters;
+                        const int per = 2 * d + 6;
+                        j.result = alt ? (const float*)j.fld[2 + 2 * per] : (const float*)j.fld[((last >> 1) + 1) & 1];
+                    } else {
+                        j.result = (const float*)j.fld[(last + 1) & 1];
+                    }
+                    break;
             }
             j.info[0] = (float)done;
             j.info[1] = c[i].re;

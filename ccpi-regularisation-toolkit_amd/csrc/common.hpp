@@ -61,6 +61,20 @@ __device__ inline T ldT(const S* p, size_t i) { return (T)Store<S>::ld(p, i); }
 template <typename T, typename S>
 __device__ inline void stT(S* p, size_t i, T v) { Store<S>::st(p, i, v); }
 
+// A value as it reads back after a store in S (identity for S = T): what a
+// later kernel sees of a field this one stores, for passes that keep using
+// the value themselves.
+template <typename T, typename S>
+__device__ inline T through_storage(T v) {
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        return (T)__uint_as_float(((uint32_t)Store<bf16_t>::bits_of((float)v)) << 16);
+    } else if constexpr (std::is_same<S, _Float16>::value) {
+        return (T)(float)(_Float16)(float)v;
+    } else {
+        return v;
+    }
+}
+
 // Element access with an index type I.  With I = uint32_t the byte offset is
 // formed in 32 bits (the host guarantees it fits), so loads use the
 // scalar-base + 32-bit vector-offset addressing mode instead of 64-bit
@@ -79,28 +93,43 @@ template <> struct IdxT<uint64_t> {
     template <typename S> __device__ static inline S* at(S* p, uint64_t e) { return p + e; }
 };
 
+// Every field lives in device global memory: the accesses go through
+// global-address-space pointers, so they compile to global (not flat)
+// memory instructions even where the pointer was read from memory (the
+// grouped kernels' job descriptors) rather than passed as a kernel argument.
+template <typename S>
+__device__ inline const __attribute__((address_space(1))) S* gmem(const S* p) {
+    return (const __attribute__((address_space(1))) S*)p;
+}
+template <typename S>
+__device__ inline __attribute__((address_space(1))) S* gmem(S* p) {
+    return (__attribute__((address_space(1))) S*)p;
+}
+
 template <typename T, typename I, typename S>
 __device__ inline T ldx(const S* p, I e) {
     if constexpr (std::is_same<S, bf16_t>::value) {
-        return (T)__uint_as_float(((uint32_t)IdxT<I>::at(p, e)->bits) << 16);
+        return (T)__uint_as_float(((uint32_t)gmem(IdxT<I>::at(p, e))->bits) << 16);
     } else {
-        return (T)*IdxT<I>::at(p, e);
+        return (T)*gmem(IdxT<I>::at(p, e));
     }
 }
 template <typename T, typename I, typename S>
 __device__ inline void stx(S* p, I e, T v) {
-    Store<S>::st(IdxT<I>::at(p, e), 0, v);
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        gmem(IdxT<I>::at(p, e))->bits = Store<bf16_t>::bits_of((float)v);
+    } else if constexpr (std::is_same<S, _Float16>::value) {
+        *gmem(IdxT<I>::at(p, e)) = (_Float16)(float)v;
+    } else {
+        *gmem(IdxT<I>::at(p, e)) = (S)v;
+    }
 }
-
-// Non-temporal store of a field the current kernel never reads back: the
-// line is marked for early eviction, so the output stream does not push the
-// inputs' halo and z-neighbour lines out of L2 before they are reused.
 template <typename T, typename I, typename S>
 __device__ inline void stnt(S* p, I e, T v) {
     if constexpr (std::is_same<S, bf16_t>::value) {
-        __builtin_nontemporal_store(Store<bf16_t>::bits_of((float)v), &IdxT<I>::at(p, e)->bits);
+        __builtin_nontemporal_store(Store<bf16_t>::bits_of((float)v), &gmem(IdxT<I>::at(p, e))->bits);
     } else {
-        __builtin_nontemporal_store((S)v, IdxT<I>::at(p, e));
+        __builtin_nontemporal_store((S)v, gmem(IdxT<I>::at(p, e)));
     }
 }
 
@@ -210,11 +239,15 @@ template <> struct Mixed<float> {
     __device__ static inline float rof_sqrt_d(float s) { return (float)sqrt((double)s + 1.0e-8); }
     // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
     __device__ static inline float pd_div(float num, float lt) { return (float)((double)num / (1.0 + (double)lt)); }
-    // squared minmod, ROF_TV_core.c:132-133 (fabs promotes to double, exact here)
+    // squared minmod, ROF_TV_core.c:132-133.  The reference multiplies in
+    // double (fabs promotes) and rounds the product to float; the factor is
+    // 0, +-0.5 or +-1, so the double product is exact and its rounding to
+    // float equals the correctly rounded float product: float arithmetic
+    // gives the same bits.
     __device__ static inline float minmod_sq(float a, float b) {
         int sa = (a > 0.f) - (a < 0.f), sb = (b > 0.f) - (b < 0.f);
-        double fa = fabs((double)a), fb = fabs((double)b);
-        float m = (float)((0.5f * (float)(sa + sb)) * (fa < fb ? fa : fb));
+        const float fa = fabsf(a), fb = fabsf(b);
+        const float m = (0.5f * (float)(sa + sb)) * (fa < fb ? fa : fb);
         return m * m;
     }
 };

@@ -191,17 +191,7 @@ __global__ __launch_bounds__(TX * TY) void fgp_iter_kernel(FgpArgs<T, S> a) {
 //                      R_{k+1},3(zi - 2)
 //   S4  P_{k+2}(zi-2)  on T
 // P_{k+1} is rounded through the storage type before it enters R_{k+1}, as
-// the one-iteration kernel reads it back from memory.
-template <typename T, typename S>
-__device__ inline T through_storage(T v) {
-    if constexpr (std::is_same<S, bf16_t>::value) {
-        return (T)__uint_as_float(((uint32_t)Store<bf16_t>::bits_of((float)v)) << 16);
-    } else if constexpr (std::is_same<S, _Float16>::value) {
-        return (T)(float)(_Float16)(float)v;
-    } else {
-        return v;
-    }
-}
+// the one-iteration kernel reads it back from memory (through_storage).
 
 template <typename T, typename S>
 struct Fgp2Args {

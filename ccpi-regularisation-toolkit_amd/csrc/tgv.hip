@@ -33,6 +33,35 @@ static void launch_tgv(const TgvArgs<T, S>& a, bool small, hipStream_t s) {
     else hipLaunchKernelGGL((tgv_iter_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, s, a);
 }
 
+template <int TY, typename T, typename S>
+static const void* tgv2_kernel_ptr(bool small) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (small) return (const void*)tgv2_iter_kernel<TY, T, S>;
+        return (const void*)tgv2_iter_kernel_uncapped<TY, T, S, uint64_t>;
+    } else {
+        return small ? (const void*)tgv2_iter_kernel_uncapped<TY, T, S, uint32_t>
+                     : (const void*)tgv2_iter_kernel_uncapped<TY, T, S, uint64_t>;
+    }
+}
+
+template <int TY, typename T, typename S>
+static void launch_tgv2(const Tgv2Args<T, S>& a, bool small, hipStream_t s) {
+    dim3 grid(a.g.nblocks), block(TX, TY);
+    if constexpr (std::is_same<T, float>::value) {
+        if (small) hipLaunchKernelGGL((tgv2_iter_kernel<TY, T, S>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((tgv2_iter_kernel_uncapped<TY, T, S, uint64_t>), grid, block, 0, s, a);
+    } else {
+        if (small) hipLaunchKernelGGL((tgv2_iter_kernel_uncapped<TY, T, S, uint32_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((tgv2_iter_kernel_uncapped<TY, T, S, uint64_t>), grid, block, 0, s, a);
+    }
+}
+
+// Device-resident TGV solve.  In 3D, iterations run two per pass
+// (tgv2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
+// 2D iteration, runs on the one-iteration kernel.  Each pass reads state set
+// `cur` and writes set cur ^ 1; a pass whose first iteration is checked also
+// writes u_bar after that iteration to Ualt, the output should the stopping
+// rule fire there.
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re) {
@@ -48,9 +77,13 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     const int nq = dim == 3 ? 6 : 3;
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
+    else if (width == 2)
+        g = ty == 8 ? plan_geom(tgv2_kernel_ptr<8, T, S>(small), TX * 8, nx, ny, nz, 8, opts.zchunk, 2.5)
+                    : plan_geom(tgv2_kernel_ptr<4, T, S>(small), TX * 4, nx, ny, nz, 4, opts.zchunk, 2.5);
     else if (ty == 8)
         g = plan_geom(small ? (const void*)tgv_iter_kernel<3, 8, T, S, uint32_t> : (const void*)tgv_iter_kernel<3, 8, T, S, uint64_t>,
                       TX * 8, nx, ny, nz, 8, opts.zchunk);
@@ -59,15 +92,17 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
 
     Arena ar;
-    size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
+    size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oAlt = 0, oCtrl, oPart;
     for (int b = 0; b < 2; ++b) {
         oU[b] = ar.reserve(n * sizeof(T));
         for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
         for (int c = 0; c < nq; ++c) oQ[b][c] = ar.reserve(n * sizeof(S));
     }
+    const bool use_tol = epsil != (T)0;
+    if (width == 2 && use_tol) oAlt = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
-    oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
+    oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
     launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
     for (int c = 0; c < dim; ++c) {
@@ -76,49 +111,96 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     }
     for (int c = 0; c < nq; ++c) launch_zero(ar.at(oQ[0][c]), n * sizeof(S), s);
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
-    double* partials = (double*)ar.at(oPart);
+    double* partials0 = (double*)ar.at(oPart);
+    double* partials1 = partials0 + 2 * (size_t)g.nblocks;
     const T tau = (T)pow((double)L2, -0.5);   // TGV_core.c:54-55
 
-    auto step = [&](int k, bool check) {
-        TgvArgs<T, S> a;
-        const int cur = k & 1, nxt = cur ^ 1;
-        a.f = d_f;
-        a.U = (const T*)ar.at(oU[cur]);
-        a.Uout = (T*)ar.at(oU[nxt]);
-        for (int c = 0; c < 3; ++c) {
-            const int cc = c < dim ? c : 0;
-            a.V[c] = (const S*)ar.at(oV[cur][cc]);
-            a.Vout[c] = (S*)ar.at(oV[nxt][cc]);
-            a.P[c] = (const S*)ar.at(oP[cur][cc]);
-            a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
-        }
-        for (int c = 0; c < 6; ++c) {
-            const int cc = c < nq ? c : 0;
-            a.Q[c] = (const S*)ar.at(oQ[cur][cc]);
-            a.Qout[c] = (S*)ar.at(oQ[nxt][cc]);
-        }
-        a.lambda = lambda;
-        a.tau = tau;
-        a.sigma = tau;
-        a.alpha1 = alpha1;
-        a.alpha0 = alpha0;
-        a.check = check;
-        a.partials = partials;
-        a.ctrl = ctrl;
-        a.g = g;
-        if (dim == 3) {
-            if (ty == 8) launch_tgv<3, 8>(a, small, s);
-            else launch_tgv<3, 4>(a, small, s);
+    int cur = 0;
+    std::vector<int> pass_k, pass_n, pass_out;   // where each pass left its state
+    auto step = [&](int k, int nk, bool c0, bool c1) {
+        const int nxt = cur ^ 1;
+        pass_k.push_back(k);
+        pass_n.push_back(nk);
+        pass_out.push_back(nxt);
+        if (nk == 2) {
+            Tgv2Args<T, S> a;
+            a.f = d_f;
+            a.U = (const T*)ar.at(oU[cur]);
+            a.Uout = (T*)ar.at(oU[nxt]);
+            for (int c = 0; c < 3; ++c) {
+                a.V[c] = (const S*)ar.at(oV[cur][c]);
+                a.Vout[c] = (S*)ar.at(oV[nxt][c]);
+                a.P[c] = (const S*)ar.at(oP[cur][c]);
+                a.Pout[c] = (S*)ar.at(oP[nxt][c]);
+            }
+            for (int c = 0; c < 6; ++c) {
+                a.Q[c] = (const S*)ar.at(oQ[cur][c]);
+                a.Qout[c] = (S*)ar.at(oQ[nxt][c]);
+            }
+            a.Ualt = use_tol ? (T*)ar.at(oAlt) : nullptr;
+            a.lambda = lambda;
+            a.tau = tau;
+            a.sigma = tau;
+            a.alpha1 = alpha1;
+            a.alpha0 = alpha0;
+            a.check0 = c0;
+            a.check1 = c1;
+            a.partials0 = partials0;
+            a.partials1 = partials1;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (ty == 8) launch_tgv2<8>(a, small, s);
+            else launch_tgv2<4>(a, small, s);
         } else {
-            launch_tgv<2, 4>(a, small, s);
+            TgvArgs<T, S> a;
+            a.f = d_f;
+            a.U = (const T*)ar.at(oU[cur]);
+            a.Uout = (T*)ar.at(oU[nxt]);
+            for (int c = 0; c < 3; ++c) {
+                const int cc = c < dim ? c : 0;
+                a.V[c] = (const S*)ar.at(oV[cur][cc]);
+                a.Vout[c] = (S*)ar.at(oV[nxt][cc]);
+                a.P[c] = (const S*)ar.at(oP[cur][cc]);
+                a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
+            }
+            for (int c = 0; c < 6; ++c) {
+                const int cc = c < nq ? c : 0;
+                a.Q[c] = (const S*)ar.at(oQ[cur][cc]);
+                a.Qout[c] = (S*)ar.at(oQ[nxt][cc]);
+            }
+            a.lambda = lambda;
+            a.tau = tau;
+            a.sigma = tau;
+            a.alpha1 = alpha1;
+            a.alpha0 = alpha0;
+            a.check = c0;
+            a.partials = partials0;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (dim == 3) {
+                if (ty == 8) launch_tgv<3, 8>(a, small, s);
+                else launch_tgv<3, 4>(a, small, s);
+            } else {
+                launch_tgv<2, 4>(a, small, s);
+            }
         }
+        cur = nxt;
     };
     int done = 0;
-    if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
+    if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials0, partials1, g.nblocks, s, step, &done, re) != 0)
+        return -1;
     *iters_done = done;
-    // the state after iteration `done` (stop) or `iter - 1` lives in buffer (last + 1) & 1
+    // the output is u_bar after iteration `last`: the stop iteration, or iter - 1
     const int last = (done < iter) ? done : iter - 1;
-    CILREG_CK(hipMemcpyAsync(d_out, ar.at(oU[(last + 1) & 1]), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    const void* src = nullptr;
+    for (size_t q = 0; q < pass_k.size(); ++q) {
+        if (last >= pass_k[q] && last < pass_k[q] + pass_n[q]) {
+            src = (pass_n[q] == 2 && last == pass_k[q]) ? ar.at(oAlt) : ar.at(oU[pass_out[q]]);
+            break;
+        }
+    }
+    if (!src) { set_error("TGV: no pass covers iteration %d", last); return -1; }
+    CILREG_CK(hipMemcpyAsync(d_out, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }

@@ -30,50 +30,24 @@ struct TgvArgs {
 // and the +y neighbour (along y).
 enum { L_P1, L_P2, L_Q11, L_Q22, L_Q12, L_Q13, L_Q23, L_N };
 
-// Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
-// the point) are passed in when the caller carries them from the previous
-// plane (carried = true) and loaded otherwise; uz / vz return u_bar and v_bar
-// one plane up, which are the next plane's own values.  Every load is
-// unconditional: neighbour indices are clamped at the last index and the
-// boundary rule is applied with selects, so the loads issue back to back.
-template <int DIM, typename I, typename T, typename S>
-__device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, bool carried, T& u, T v[3],
-                                   T& uz, T vz[3], T p[3], T q[6]) {
-    const Geom& g = a.g;
-    const I sxy = (I)g.sxy, nx = (I)g.nx;
-    const I i = (I)z * sxy + (I)py * nx + (I)px;
-    const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
-    const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
-    const T sigma = a.sigma;
-    if (!carried) {
-        u = ldx<T>(a.U, i);
-        #pragma unroll
-        for (int c = 0; c < DIM; ++c) v[c] = ldx<T>(a.V[c], i);
-    }
-    const T ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
-    uz = DIM == 3 ? ldx<T>(a.U, iz) : (T)0;
-    T vx[3], vy[3], po[3], qo[6];
-    #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        vx[c] = ldx<T>(a.V[c], ix);
-        vy[c] = ldx<T>(a.V[c], iy);
-        if (DIM == 3) vz[c] = ldx<T>(a.V[c], iz);
-        po[c] = ldx<T>(a.P[c], i);
-    }
-    const int nq = DIM == 3 ? 6 : 3;
-    #pragma unroll
-    for (int c = 0; c < nq; ++c) qo[c] = ldx<T>(a.Q[c], i);
-
+// The dual step at one point (TGV_core.c:351-446): P and Q of the next
+// iterate from the old P, Q there, u_bar and v_bar there (u, v) and at the
+// +x / +y / +z neighbours, and whether the point is the last along each axis
+// (the forward difference is zero there).  Shared by every TGV kernel.
+template <int DIM, typename T>
+__device__ inline void tgv_dual_math(T sigma, T alpha1, T alpha0, bool lx, bool ly, bool lz, T u, const T v[3], T ux,
+                                     T uy, T uz, const T vx[3], const T vy[3], const T vz[3], const T po[3],
+                                     const T qo[6], T p[3], T q[6]) {
     // P (TGV_core.c:360-365): forward difference of u_bar minus v_bar
     p[0] = po[0] + sigma * (lx ? -v[0] : (ux - u) - v[0]);
     p[1] = po[1] + sigma * (ly ? -v[1] : (uy - u) - v[1]);
     T nrm;
     if (DIM == 3) {
         p[2] = po[2] + sigma * (lz ? -v[2] : (uz - u) - v[2]);
-        nrm = dsqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / a.alpha1;
+        nrm = dsqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / alpha1;
     } else {
         p[2] = 0;
-        nrm = dsqrt(p[0] * p[0] + p[1] * p[1]) / a.alpha1;
+        nrm = dsqrt(p[0] * p[0] + p[1] * p[1]) / alpha1;
     }
     if (nrm > (T)1) {
         p[0] /= nrm;
@@ -99,7 +73,7 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
         q[5] = qo[5] + sigma * ((T)0.5 * (d[2][1] + d[1][2]));
         T m = dsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + (T)2 * q[3] * q[3] + (T)2 * q[4] * q[4] +
                     (T)2 * q[5] * q[5]);
-        m = m / a.alpha0;
+        m = m / alpha0;
         if (m > (T)1) {
             #pragma unroll
             for (int c = 0; c < 6; ++c) q[c] /= m;
@@ -110,13 +84,48 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
         q[2] = qo[2] + sigma * ((T)0.5 * (d[1][0] + d[0][1]));
         q[3] = q[4] = q[5] = 0;
         T m = dsqrt(q[0] * q[0] + q[1] * q[1] + (T)2 * q[2] * q[2]);
-        m = m / a.alpha0;
+        m = m / alpha0;
         if (m > (T)1) {
             q[0] /= m;
             q[1] /= m;
             q[2] /= m;
         }
     }
+}
+
+// Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
+// the point) are passed in when the caller carries them from the previous
+// plane (carried = true) and loaded otherwise; uz / vz return u_bar and v_bar
+// one plane up, which are the next plane's own values.  Every load is
+// unconditional: neighbour indices are clamped at the last index and the
+// boundary rule is applied with selects, so the loads issue back to back.
+template <int DIM, typename I, typename T, typename S>
+__device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z, bool carried, T& u, T v[3],
+                                   T& uz, T vz[3], T p[3], T q[6]) {
+    const Geom& g = a.g;
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const I i = (I)z * sxy + (I)py * nx + (I)px;
+    const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
+    const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
+    if (!carried) {
+        u = ldx<T>(a.U, i);
+        #pragma unroll
+        for (int c = 0; c < DIM; ++c) v[c] = ldx<T>(a.V[c], i);
+    }
+    const T ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
+    uz = DIM == 3 ? ldx<T>(a.U, iz) : (T)0;
+    T vx[3] = {0, 0, 0}, vy[3] = {0, 0, 0}, po[3] = {0, 0, 0}, qo[6] = {0, 0, 0, 0, 0, 0};
+    #pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        vx[c] = ldx<T>(a.V[c], ix);
+        vy[c] = ldx<T>(a.V[c], iy);
+        if (DIM == 3) vz[c] = ldx<T>(a.V[c], iz);
+        po[c] = ldx<T>(a.P[c], i);
+    }
+    const int nq = DIM == 3 ? 6 : 3;
+    #pragma unroll
+    for (int c = 0; c < nq; ++c) qo[c] = ldx<T>(a.Q[c], i);
+    tgv_dual_math<DIM, T>(a.sigma, a.alpha1, a.alpha0, lx, ly, lz, u, v, ux, uy, uz, vx, vy, vz, po, qo, p, q);
 }
 
 // adjoint difference: own value at index 0, minus the predecessor at the last
@@ -126,6 +135,37 @@ __device__ inline T bdiff(T own, T pred, int pos, int len) {
     if (pos == 0) return own;
     if (pos == len - 1) return -pred;
     return own - pred;
+}
+
+// The primal step at one point (TGV_core.c:448-530): u_bar and v_bar of the
+// next iterate from the new P, Q there (p, q), their -x / -y / -z
+// neighbours (P1, Q11, Q12, Q13 at x-1; P2, Q12, Q22, Q23 at y-1; P3, Q13,
+// Q23, Q33 at z-1), the old u_bar, v_bar (ub, vb) and f.  Shared by every
+// TGV kernel.
+template <int DIM, typename T>
+__device__ inline void tgv_primal_math(T lambda, T tau, int x, int y, int z, int nx, int ny, int nz, const T p[3],
+                                       const T q[6], const T xm[4], const T ym[4], const T zm[4], T ub,
+                                       const T vb[3], T f, T& ubn, T vbn[3]) {
+    // xm = {P1, Q11, Q12, Q13}(x-1), ym = {P2, Q12, Q22, Q23}(y-1), zm = {P3, Q13, Q23, Q33}(z-1)
+    T dv = bdiff(p[0], xm[0], x, nx) + bdiff(p[1], ym[0], y, ny);
+    if (DIM == 3) dv = dv + bdiff(p[2], zm[0], z, nz);
+    const T un = (lambda * (ub + tau * dv) + tau * f) / (lambda + tau);
+    ubn = (T)2 * un - ub;
+    T dq[3];
+    if (DIM == 3) {
+        dq[0] = bdiff(q[0], xm[1], x, nx) + bdiff(q[3], ym[1], y, ny) + bdiff(q[4], zm[1], z, nz);
+        dq[1] = bdiff(q[3], xm[2], x, nx) + bdiff(q[1], ym[2], y, ny) + bdiff(q[5], zm[2], z, nz);
+        dq[2] = bdiff(q[4], xm[3], x, nx) + bdiff(q[5], ym[3], y, ny) + bdiff(q[2], zm[3], z, nz);
+    } else {
+        dq[0] = bdiff(q[0], xm[1], x, nx) + bdiff(q[2], ym[1], y, ny);
+        dq[1] = bdiff(q[2], xm[2], x, nx) + bdiff(q[1], ym[2], y, ny);
+        dq[2] = 0;
+    }
+    #pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        const T vn = vb[c] + tau * (p[c] + dq[c]);
+        vbn[c] = (T)2 * vn - vb[c];
+    }
 }
 
 template <int DIM, int TY, typename T, typename S, typename I>
@@ -203,35 +243,32 @@ __device__ inline void tgv_body(const TgvArgs<T, S>& a, const TileCoord& tc) {
         if (own) {
             const I i = (I)z * sxy + (I)y * nx + (I)x;
             // x-predecessor at [ly+1][lx], y-predecessor at [ly][lx+1]
-            T dv = bdiff(p[0], L[L_P1][ly + 1][lx], x, g.nx) + bdiff(p[1], L[L_P2][ly][lx + 1], y, g.ny);
-            if (DIM == 3) dv = dv + bdiff(p[2], zp3, z, g.nz);
+            T xm[4], ym[4], zm[4];
+            xm[0] = L[L_P1][ly + 1][lx];
+            ym[0] = L[L_P2][ly][lx + 1];
+            zm[0] = zp3;
+            if (DIM == 3) {
+                xm[1] = L[L_Q11][ly + 1][lx]; xm[2] = L[L_Q12][ly + 1][lx]; xm[3] = L[L_Q13][ly + 1][lx];
+                ym[1] = L[L_Q12][ly][lx + 1]; ym[2] = L[L_Q22][ly][lx + 1]; ym[3] = L[L_Q23][ly][lx + 1];
+                zm[1] = zq13; zm[2] = zq23; zm[3] = zq33;
+            } else {
+                xm[1] = L[L_Q11][ly + 1][lx]; xm[2] = L[L_Q12][ly + 1][lx]; xm[3] = 0;
+                ym[1] = L[L_Q12][ly][lx + 1]; ym[2] = L[L_Q22][ly][lx + 1]; ym[3] = 0;
+                zm[1] = zm[2] = zm[3] = 0;
+            }
             const T ub = cu;
-            const T un = (a.lambda * (ub + a.tau * dv) + a.tau * ldx<T>(a.f, i)) / (a.lambda + a.tau);
-            const T ubn = (T)2 * un - ub;
+            T ubn, vbn[3] = {0, 0, 0};
+            tgv_primal_math<DIM, T>(a.lambda, a.tau, x, y, z, g.nx, g.ny, g.nz, p, q, xm, ym, zm, ub, cv,
+                                    ldx<T>(a.f, i), ubn, vbn);
             if (a.check) {
                 const T df = ubn - ub;
                 sdd += (double)df * (double)df;
                 suu += (double)ubn * (double)ubn;
             }
             stnt<T>(a.Uout, i, ubn);
-            T dq[3];
-            if (DIM == 3) {
-                dq[0] = bdiff(q[0], L[L_Q11][ly + 1][lx], x, g.nx) + bdiff(q[3], L[L_Q12][ly][lx + 1], y, g.ny) +
-                        bdiff(q[4], zq13, z, g.nz);
-                dq[1] = bdiff(q[3], L[L_Q12][ly + 1][lx], x, g.nx) + bdiff(q[1], L[L_Q22][ly][lx + 1], y, g.ny) +
-                        bdiff(q[5], zq23, z, g.nz);
-                dq[2] = bdiff(q[4], L[L_Q13][ly + 1][lx], x, g.nx) + bdiff(q[5], L[L_Q23][ly][lx + 1], y, g.ny) +
-                        bdiff(q[2], zq33, z, g.nz);
-            } else {
-                dq[0] = bdiff(q[0], L[L_Q11][ly + 1][lx], x, g.nx) + bdiff(q[2], L[L_Q12][ly][lx + 1], y, g.ny);
-                dq[1] = bdiff(q[2], L[L_Q12][ly + 1][lx], x, g.nx) + bdiff(q[1], L[L_Q22][ly][lx + 1], y, g.ny);
-                dq[2] = 0;
-            }
             #pragma unroll
             for (int c = 0; c < DIM; ++c) {
-                const T vb = cv[c];
-                const T vn = vb + a.tau * (p[c] + dq[c]);
-                stnt<T>(a.Vout[c], i, (T)2 * vn - vb);
+                stnt<T>(a.Vout[c], i, vbn[c]);
                 stnt<T>(a.Pout[c], i, p[c]);
             }
             const int nq = DIM == 3 ? 6 : 3;
@@ -258,6 +295,309 @@ template <int DIM, int TY, typename T, typename S, typename I>
 __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
     tgv_body<DIM, TY, T, S, I>(a, tile_of_block(a.g, TY));
+}
+
+// ---------------------------------------------------------------------------
+// Two iterations per pass (temporal blocking of depth 2, 3D)
+// ---------------------------------------------------------------------------
+// The TGV state (u_bar, v_bar, P, Q) does not involve older iterates, so a
+// pass that advances k -> k+2 reads the 14 input fields once and writes the
+// 13 state fields once: 108 B per two voxel-iterations instead of 216.  Each
+// value is computed by tgv_dual_math / tgv_primal_math with the inputs the
+// one-iteration kernel would use (intermediate fields rounded through the
+// storage type, as they would be stored and read back), so the results are
+// bit-identical.
+//
+// The workgroup owns the 64 x TY tile T and evaluates iteration k on
+// B = [x0-2, x0+64] x [y0-2, y0+TY] (67 x (TY+3) points; slot 0 of every
+// thread is its own point of T, slot 1 of 3*67 + 3*TY ring lanes a point of
+// B \ T).  Validity shrinks stage by stage: the duals of iteration k on B,
+// its primal from x0-1 / y0-1 on, the duals of k+1 on [x0-1, x0+63] x
+// [y0-1, y0+TY-1], the primal of k+1 on T.
+//
+// z pipeline for plane zi of the march over [z0 - 2, z1]:
+//   S1  P, Q of k+1 at zi        (u_bar, v_bar of k at zi and zi+1)
+//   S2  u_bar, v_bar of k+1 at zi (P, Q of k+1 at zi, x-1, y-1 and zi-1)
+//   S3  P, Q of k+2 at zi-1      (u_bar, v_bar of k+1 at zi-1, +x, +y, zi)
+//   S4  u_bar, v_bar of k+2 at zi-1 on T; all 13 state fields stored.
+template <typename T, typename S>
+struct Tgv2Args {
+    const T* f;
+    const T* U;
+    const S* V[3];
+    const S* P[3];
+    const S* Q[6];
+    T* Uout;
+    S* Vout[3];
+    S* Pout[3];
+    S* Qout[6];
+    T* Ualt;            // u_bar after iteration k (written when check0: the output of a stop at k)
+    T lambda, tau, sigma, alpha1, alpha0;
+    int check0, check1;
+    double* partials0;
+    double* partials1;
+    const Ctrl* ctrl;
+    Geom g;
+};
+
+template <int TY>
+struct Tgv2Shape {
+    static constexpr int BW = TX + 3, BH = TY + 3;
+    static constexpr int NR = 3 * BW + 3 * TY;
+    static constexpr int NT = TX * TY;
+};
+
+template <int TY, typename T, typename S, typename I>
+__device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
+    using Sh = Tgv2Shape<TY>;
+    constexpr int BW = Sh::BW, BH = Sh::BH, NR = Sh::NR, NT = Sh::NT;
+    const Geom& g = a.g;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
+
+    int bx[2], by[2];
+    bx[0] = lx + 2;
+    by[0] = ly + 2;
+    const int hbase = (tc.lin & 1) ? NT - NR : 0;
+    const int h = tid - hbase;
+    const bool ring = h >= 0 && h < NR;
+    {
+        int rx = 0, ry = 0;
+        if (h < 3 * BW) {
+            const int row = h / BW;
+            rx = h - row * BW;
+            ry = row < 2 ? row : TY + 2;           // rows y0-2, y0-1, y0+TY
+        } else {
+            const int k = h - 3 * BW, col = k / TY;
+            rx = col < 2 ? col : TX + 2;           // columns x0-2, x0-1, x0+TX
+            ry = k - col * TY + 2;
+        }
+        bx[1] = ring ? rx : 0;
+        by[1] = ring ? ry : 0;
+    }
+    const int nslot = ring ? 2 : 1;
+    int X[2], Y[2], cx[2], cy[2];
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        X[s] = tc.x0 + bx[s] - 2;
+        Y[s] = tc.y0 + by[s] - 2;
+        const bool in = X[s] >= 0 && Y[s] >= 0 && X[s] < g.nx && Y[s] < g.ny;
+        cx[s] = in ? X[s] : 0;
+        cy[s] = in ? Y[s] : 0;
+    }
+    const bool own = X[0] < g.nx && Y[0] < g.ny;
+
+    __shared__ T PQ[L_N][BH + 1][BW + 1];      // P, Q channels read at x-1 / y-1
+    __shared__ T UV[2][4][BH + 1][BW + 1];     // u_bar, v_bar of k+1, planes by parity
+    __shared__ double red[8];
+
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;
+    const T sigma = a.sigma, tau = a.tau, lam = a.lambda;
+
+    // views of the input state for tgv_dual_at
+    TgvArgs<T, S> in;
+    in.f = a.f;
+    in.U = a.U;
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) { in.V[c] = a.V[c]; in.P[c] = a.P[c]; }
+    #pragma unroll
+    for (int c = 0; c < 6; ++c) in.Q[c] = a.Q[c];
+    in.sigma = sigma;
+    in.alpha1 = a.alpha1;
+    in.alpha0 = a.alpha0;
+    in.g = g;
+
+    T cu[2] = {0, 0}, cv[2][3] = {};     // u_bar, v_bar of k at the current plane (carried)
+    T p1v[2][3] = {}, q1v[2][6] = {};    // P, Q of k+1 at the previous plane
+    T zp2[4] = {0, 0, 0, 0};             // P3, Q13, Q23, Q33 of k+2 at the previous plane (slot 0)
+    T f0 = 0;                            // f at the previous plane (slot 0)
+    double sdd0 = 0.0, suu0 = 0.0, sdd1 = 0.0, suu1 = 0.0;
+    const int zs = z0 >= 2 ? z0 - 2 : 0;
+    for (int zi = zs; zi <= z1; ++zi) {
+        const bool s12 = zi < nz;
+        T p1[2][3] = {}, q1[2][6] = {}, nu[2] = {0, 0}, nv[2][3] = {};
+        // ---- S1: P, Q of k+1 at zi
+        if (s12) {
+            #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    tgv_dual_at<3, I>(in, cx[s], cy[s], zi, zi > zs, cu[s], cv[s], nu[s], nv[s], p1[s], q1[s]);
+                    PQ[L_P1][by[s]][bx[s]] = p1[s][0];
+                    PQ[L_P2][by[s]][bx[s]] = p1[s][1];
+                    PQ[L_Q11][by[s]][bx[s]] = q1[s][0];
+                    PQ[L_Q22][by[s]][bx[s]] = q1[s][1];
+                    PQ[L_Q12][by[s]][bx[s]] = q1[s][3];
+                    PQ[L_Q13][by[s]][bx[s]] = q1[s][4];
+                    PQ[L_Q23][by[s]][bx[s]] = q1[s][5];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S2: u_bar, v_bar of k+1 at zi
+        T fcur = 0;
+        if (s12 && zi + 1 >= z0) {
+            #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const int xb = bx[s] > 0 ? bx[s] - 1 : 0, yb = by[s] > 0 ? by[s] - 1 : 0;
+                    T xm[4], ym[4], zm[4];
+                    xm[0] = PQ[L_P1][by[s]][xb];
+                    xm[1] = PQ[L_Q11][by[s]][xb];
+                    xm[2] = PQ[L_Q12][by[s]][xb];
+                    xm[3] = PQ[L_Q13][by[s]][xb];
+                    ym[0] = PQ[L_P2][yb][bx[s]];
+                    ym[1] = PQ[L_Q12][yb][bx[s]];
+                    ym[2] = PQ[L_Q22][yb][bx[s]];
+                    ym[3] = PQ[L_Q23][yb][bx[s]];
+                    zm[0] = p1v[s][2];
+                    zm[1] = q1v[s][4];
+                    zm[2] = q1v[s][5];
+                    zm[3] = q1v[s][2];
+                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    const T fv = ldx<T>(a.f, i);
+                    if (s == 0) fcur = fv;
+                    T ubn, vbn[3];
+                    tgv_primal_math<3, T>(lam, tau, X[s], Y[s], zi, g.nx, g.ny, nz, p1[s], q1[s], xm, ym, zm,
+                                          cu[s], cv[s], fv, ubn, vbn);
+                    if (s == 0 && own && zi >= z0 && zi < z1) {
+                        if (a.check0) {
+                            const T df = ubn - cu[0];
+                            sdd0 += (double)df * (double)df;
+                            suu0 += (double)ubn * (double)ubn;
+                            stnt<T>(a.Ualt, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], ubn);
+                        }
+                    }
+                    UV[zi & 1][0][by[s]][bx[s]] = ubn;
+                    #pragma unroll
+                    for (int c = 0; c < 3; ++c) UV[zi & 1][1 + c][by[s]][bx[s]] = through_storage<T, S>(vbn[c]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S3: P, Q of k+2 at zc
+        const int zc = zi - 1;
+        T p2[2][3] = {}, q2[2][6] = {};
+        const bool s3 = zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
+        if (s3) {
+            #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const int b = zc & 1;
+                    const int xp = bx[s] + 1, yp = by[s] + 1;
+                    const T u = UV[b][0][by[s]][bx[s]];
+                    const T ux = UV[b][0][by[s]][xp], uy = UV[b][0][yp][bx[s]], uz = UV[b ^ 1][0][by[s]][bx[s]];
+                    T v[3], vx[3], vy[3], vz[3], po[3], qo[6];
+                    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        v[c] = UV[b][1 + c][by[s]][bx[s]];
+                        vx[c] = UV[b][1 + c][by[s]][xp];
+                        vy[c] = UV[b][1 + c][yp][bx[s]];
+                        vz[c] = UV[b ^ 1][1 + c][by[s]][bx[s]];
+                        po[c] = through_storage<T, S>(p1v[s][c]);
+                    }
+                    #pragma unroll
+                    for (int c = 0; c < 6; ++c) qo[c] = through_storage<T, S>(q1v[s][c]);
+                    const bool bl_x = X[s] == g.nx - 1, bl_y = Y[s] == g.ny - 1, bl_z = zc == nz - 1;
+                    tgv_dual_math<3, T>(sigma, a.alpha1, a.alpha0, bl_x, bl_y, bl_z, u, v, ux, uy, uz, vx, vy, vz,
+                                        po, qo, p2[s], q2[s]);
+                    PQ[L_P1][by[s]][bx[s]] = p2[s][0];
+                    PQ[L_P2][by[s]][bx[s]] = p2[s][1];
+                    PQ[L_Q11][by[s]][bx[s]] = q2[s][0];
+                    PQ[L_Q22][by[s]][bx[s]] = q2[s][1];
+                    PQ[L_Q12][by[s]][bx[s]] = q2[s][3];
+                    PQ[L_Q13][by[s]][bx[s]] = q2[s][4];
+                    PQ[L_Q23][by[s]][bx[s]] = q2[s][5];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S4: u_bar, v_bar of k+2 at zc on T; store the state
+        if (s3 && zc >= z0 && own) {
+            const int b = zc & 1;
+            const int xb = bx[0] - 1, yb = by[0] - 1;
+            T xm[4], ym[4];
+            xm[0] = PQ[L_P1][by[0]][xb];
+            xm[1] = PQ[L_Q11][by[0]][xb];
+            xm[2] = PQ[L_Q12][by[0]][xb];
+            xm[3] = PQ[L_Q13][by[0]][xb];
+            ym[0] = PQ[L_P2][yb][bx[0]];
+            ym[1] = PQ[L_Q12][yb][bx[0]];
+            ym[2] = PQ[L_Q22][yb][bx[0]];
+            ym[3] = PQ[L_Q23][yb][bx[0]];
+            const T ub = UV[b][0][by[0]][bx[0]];
+            T vb[3];
+            #pragma unroll
+            for (int c = 0; c < 3; ++c) vb[c] = UV[b][1 + c][by[0]][bx[0]];
+            T ubn, vbn[3];
+            tgv_primal_math<3, T>(lam, tau, X[0], Y[0], zc, g.nx, g.ny, nz, p2[0], q2[0], xm, ym, zp2, ub, vb, f0,
+                                  ubn, vbn);
+            if (a.check1) {
+                const T df = ubn - ub;
+                sdd1 += (double)df * (double)df;
+                suu1 += (double)ubn * (double)ubn;
+            }
+            const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
+            stnt<T>(a.Uout, io, ubn);
+            #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                stnt<T>(a.Vout[c], io, vbn[c]);
+                stnt<T>(a.Pout[c], io, p2[0][c]);
+            }
+            #pragma unroll
+            for (int c = 0; c < 6; ++c) stnt<T>(a.Qout[c], io, q2[0][c]);
+        }
+        __syncthreads();
+        // ---- shift the pipeline
+        if (s3) {
+            zp2[0] = p2[0][2];
+            zp2[1] = q2[0][4];
+            zp2[2] = q2[0][5];
+            zp2[3] = q2[0][2];
+        }
+        if (s12) {
+            #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                cu[s] = nu[s];
+                #pragma unroll
+                for (int c = 0; c < 3; ++c) { cv[s][c] = nv[s][c]; p1v[s][c] = p1[s][c]; }
+                #pragma unroll
+                for (int c = 0; c < 6; ++c) q1v[s][c] = q1[s][c];
+            }
+            f0 = fcur;
+        }
+    }
+    if (a.check0) {
+        block_sum2(sdd0, suu0, red);
+        if (tid == 0) {
+            a.partials0[2 * tc.lin] = sdd0;
+            a.partials0[2 * tc.lin + 1] = suu0;
+        }
+        __syncthreads();
+    }
+    if (a.check1) {
+        block_sum2(sdd1, suu1, red);
+        if (tid == 0) {
+            a.partials1[2 * tc.lin] = sdd1;
+            a.partials1[2 * tc.lin + 1] = suu1;
+        }
+    }
+}
+
+// FP32 arithmetic, 32-bit offsets: registers capped at 128 so two 512-thread
+// workgroups fit a CU (the FP32-storage body needs about 122; BF16 / FP16
+// storage would take 129).
+template <int TY, typename T, typename S>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_iter_kernel(Tgv2Args<T, S> a) {
+    if (solve_stopped(a.ctrl)) return;
+    tgv2_body<TY, T, S, uint32_t>(a, tile_of_block(a.g, TY));
+}
+
+// No register cap: 64-bit offsets (fields of 2 GiB and more) and FP64, whose
+// bodies do not fit 128 registers without spilling.
+template <int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) void tgv2_iter_kernel_uncapped(Tgv2Args<T, S> a) {
+    if (solve_stopped(a.ctrl)) return;
+    tgv2_body<TY, T, S, I>(a, tile_of_block(a.g, TY));
 }
 
 }  // namespace cilreg

@@ -70,3 +70,54 @@ def test_depth2_wide_index():
     x = _vol((12, 30, 66), 14)
     a, b = _pair(x, lam=0.03, iters=9, wide_index=True)
     _same(a, b, "wide index")
+
+
+# ---------------------------------------------------------------------------
+# TGV: the two-iteration pass reads the state once and writes it once; the
+# intermediate fields are rounded through the storage type as the
+# one-iteration kernel stores and re-reads them, so depth 2 == depth 1 bit
+# for bit, and the stop output (u_bar after the stop iteration, from the
+# pass's side buffer when the stop falls on its first iteration) too.
+# ---------------------------------------------------------------------------
+def _tgv_pair(x, **kw):
+    from ccpi.filters import device as D
+    a = D.tgv(x, depth=1, **kw)
+    b = D.tgv(x, depth=2, **kw)
+    return a, b
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("zchunk", [0, 1, 2, 3, 7])
+def test_tgv_depth2_matches_depth1_tilings(shape, zchunk):
+    x = _vol(shape, 21)
+    for tile_y in (4, 8):
+        for iters in (1, 2, 7, 10):
+            a, b = _tgv_pair(x, lam=0.03, alpha1=1.0, alpha0=2.0, iters=iters, zchunk=zchunk, tile_y=tile_y)
+            _same(a, b, f"TGV {shape} zchunk={zchunk} tile_y={tile_y} iters={iters}")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_tgv_depth2_matches_depth1_precisions(precision):
+    x = _vol((24, 40, 72), 22)
+    a, b = _tgv_pair(x, lam=0.02, alpha1=1.0, alpha0=2.0, iters=25, precision=precision)
+    _same(a, b, f"TGV {precision}")
+
+
+@pytest.mark.parametrize("check_every", [1, 2, 3, 5])
+def test_tgv_depth2_matches_depth1_tolerance_stops(check_every):
+    x = _vol((16, 48, 80), 23)
+    stopped = set()
+    for tol in (3e-2, 1e-2, 3e-3):
+        for zchunk in (0, 4):
+            a, b = _tgv_pair(x, lam=0.03, alpha1=1.0, alpha0=2.0, iters=150, tol=tol, check_every=check_every,
+                             zchunk=zchunk)
+            _same(a, b, f"TGV tol={tol} check_every={check_every} zchunk={zchunk}")
+            if int(a[1][0]) < 150:
+                stopped.add(int(a[1][0]) % 2)
+    assert stopped, "no tolerance stop exercised"
+
+
+def test_tgv_depth2_wide_index():
+    x = _vol((12, 30, 66), 24)
+    a, b = _tgv_pair(x, lam=0.03, alpha1=1.0, alpha0=2.0, iters=9, wide_index=True)
+    _same(a, b, "TGV wide index")
