@@ -237,16 +237,21 @@ def batched_section(D, B, torch):
 
     short = [dict(sp, iters=2) for sp in specs]
     plain(short, "fp32")   # warm-up: workspaces, code objects
-    B.batch_run(short)
-    B.batch_run(short, precision="bf16")
+    # the autotuner picks each job's z-chunk depth and pass width once per
+    # size class; tuning happens here, outside the timed runs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.batch_run(short, autotune=True)
+    B.batch_run(short, precision="bf16", autotune=True)
+    t_tune = time.perf_counter() - t0
     log("bench: batched section, plain FP32")
     ref, t_plain = timed(lambda: plain(specs, "fp32"))
     log("bench: batched section, one graph FP32")
-    got32, t_b32 = timed(lambda: B.batch_run(specs))
+    got32, t_b32 = timed(lambda: B.batch_run(specs, autotune=True))
     identical = all(bool(torch.equal(a, b)) for a, (b, _) in zip(ref, got32))
     del got32
     log("bench: batched section, one graph BF16 duals")
-    got16, t_b16 = timed(lambda: B.batch_run(specs, precision="bf16"))
+    got16, t_b16 = timed(lambda: B.batch_run(specs, precision="bf16", autotune=True))
     # BF16 error against FP64 solves of the first four FGP volumes (ROF has no dual field)
     errs = []
     for sp, (y, _) in list(zip(specs, got16))[:8]:
@@ -259,6 +264,7 @@ def batched_section(D, B, torch):
     return {
         "workload": "64 volumes, extents U{64..320}^3, half FGP-TV 300 iterations, half ROF-TV 1000 iterations",
         "voxel_iters": total_vi,
+        "autotune_seconds": round(t_tune, 3),
         "plain_fp32": {"seconds": round(t_plain, 4), "voxel_iters_per_s": total_vi / t_plain},
         "batched_graph_fp32": {"seconds": round(t_b32, 4), "voxel_iters_per_s": total_vi / t_b32,
                                "speedup_vs_plain": t_plain / t_b32, "bit_identical_to_plain": identical},

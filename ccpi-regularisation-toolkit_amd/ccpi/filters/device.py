@@ -23,15 +23,15 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 class CilregOpts(ctypes.Structure):
     _fields_ = [("check_every", ctypes.c_int), ("dual_precision", ctypes.c_int), ("tile_y", ctypes.c_int),
                 ("zchunk", ctypes.c_int), ("profile", ctypes.c_int), ("wide_index", ctypes.c_int),
-                ("stream_launches", ctypes.c_int), ("temporal_depth", ctypes.c_int)]
+                ("stream_launches", ctypes.c_int), ("temporal_depth", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0, profile=False, wide_index=False,
-              stream_launches=False, depth=0):
+              stream_launches=False, depth=0, autotune=False):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
     return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk), int(bool(profile)),
-                      int(bool(wide_index)), int(bool(stream_launches)), int(depth))
+                      int(bool(wide_index)), int(bool(stream_launches)), int(depth), int(bool(autotune)))
 
 
 def last_profile():
@@ -63,6 +63,25 @@ def stream_bench(nbytes=1 << 30, reps=20):
     if lib.cilreg_stream_bench(int(nbytes), int(reps), out) != 0:
         raise RuntimeError(f"cilreg_stream_bench failed: {last_error()}")
     return dict(zip(("copy", "read", "write", "sum2"), list(out)))
+
+
+def autotune(method, shape, method_tv=0, **kw):
+    """The autotuner's choice for `method` ('rof', 'fgp', 'pd', 'tgv') on a
+    volume of `shape` ((Z, Y, X) or (Y, X)) with the given options (e.g.
+    precision): dict(tile_y, zchunk, depth, ms_per_iter)."""
+    lib = require_gpu_lib()
+    lib.cilreg_autotune.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(CilregOpts), ctypes.POINTER(CilregOpts),
+                                                         ctypes.POINTER(ctypes.c_double)]
+    lib.cilreg_autotune.restype = ctypes.c_int
+    m = {'rof': 0, 'fgp': 1, 'pd': 2, 'tgv': 3}[method]
+    d = list(shape)[::-1] + ([1] if len(shape) == 2 else [])
+    base = make_opts(**kw)
+    best = CilregOpts()
+    ms = ctypes.c_double()
+    if lib.cilreg_autotune(m, d[0], d[1], d[2], int(method_tv), ctypes.byref(base), ctypes.byref(best),
+                           ctypes.byref(ms)) != 0:
+        raise RuntimeError(f"cilreg_autotune failed: {last_error()}")
+    return {"tile_y": best.tile_y, "zchunk": best.zchunk, "depth": best.temporal_depth, "ms_per_iter": ms.value}
 
 
 _vp = ctypes.c_void_p

@@ -20,6 +20,7 @@
 #include "fgp_kernels.hpp"
 #include "pd_kernels.hpp"
 #include "tgv_kernels.hpp"
+#include "tune.hpp"
 #include <algorithm>
 #include <cstring>
 #include <deque>
@@ -493,6 +494,7 @@ struct EJob {
     int state = J_PENDING;
     int dim = 3, nz = 1, ty = 8;
     int nbuf = 3;   // FGP dual sets: 4 with two-iteration passes (3D)
+    int width = 1;  // iterations per launch (2: 3D FGP / TGV two-iteration passes)
     bool wide = false;
     size_t n = 0, bytes = 0;
     std::unique_ptr<Arena> ar;   // every device buffer of the job, one allocation
@@ -511,14 +513,16 @@ struct EJob {
     char err[160] = {};   // why the job failed
 };
 
-// group key: method, dimensionality, projection, index width
+// group key: method, dimensionality, projection, index width, iterations
+// per launch (2 for 3D FGP / TGV unless the autotuner chose 1)
 struct GroupKey {
-    int method, dim, iso, wide;
+    int method, dim, iso, wide, width;
     bool operator<(const GroupKey& o) const {
         if (method != o.method) return method < o.method;
         if (dim != o.dim) return dim < o.dim;
         if (iso != o.iso) return iso < o.iso;
-        return wide < o.wide;
+        if (wide != o.wide) return wide < o.wide;
+        return width < o.width;
     }
 };
 
@@ -558,8 +562,17 @@ public:
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
         j->ty = j->dim == 3 ? 8 : 4;
-        j->nbuf = (j->pub.method == CILREG_FGP && j->dim == 3) ? 4 : 3;
-        const int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
+        const bool two = (j->pub.method == CILREG_FGP || j->pub.method == CILREG_TGV) && j->dim == 3;
+        int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
+        j->width = two && opts_.depth != 1 ? 2 : 1;
+        if (opts_.autotune && j->dim == 3 && opts_.zchunk == 0 && opts_.depth == 0) {
+            TuneChoice c;
+            const int iso = (j->pub.method == CILREG_FGP || j->pub.method == CILREG_PD) ? (j->pub.methodTV == 0) : 1;
+            if (autotune(j->pub.method, pj.dimX, pj.dimY, j->nz, iso, opts_, true, &c) != 0) return -1;
+            zc = c.zchunk;
+            j->width = two ? c.depth : 1;
+        }
+        j->nbuf = (j->pub.method == CILREG_FGP && j->width == 2) ? 4 : 3;
         j->g = make_geom(pj.dimX, pj.dimY, j->nz, j->ty, zc);
         plan_job(*j);
         j->bytes = j->ar->bytes();
@@ -699,7 +712,7 @@ private:
             default:                                                                       // U x2, (V, P, Q) x2
                 res(nT); res(nT);
                 for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
-                if (d == 3 && j.pub.epsil != 0.0f) res(nT);   // Ualt of two-iteration passes
+                if (j.width == 2 && j.pub.epsil != 0.0f) res(nT);   // Ualt of two-iteration passes
                 break;
         }
         j.nfld = m;
@@ -816,7 +829,7 @@ private:
 
     GroupKey key_of(const EJob& j) const {
         const int iso = (j.pub.method == CILREG_FGP || j.pub.method == CILREG_PD) ? (j.pub.methodTV == 0) : 1;
-        return GroupKey{j.pub.method, j.dim, iso, j.wide ? 1 : 0};
+        return GroupKey{j.pub.method, j.dim, iso, j.wide ? 1 : 0, j.width};
     }
 
     template <class J>
@@ -900,7 +913,7 @@ private:
                     }
                     for (int c = 0; c < 6; ++c) r.Q[b][c] = (S*)ch(base + 2 * d, c, nq);
                 }
-                r.Ualt = (d == 3 && j.pub.epsil != 0.0f) ? (T*)j.fld[2 + 2 * per] : nullptr;
+                r.Ualt = (j.width == 2 && j.pub.epsil != 0.0f) ? (T*)j.fld[2 + 2 * per] : nullptr;
                 r.partials1 = j.partials + 2 * (size_t)j.g.nblocks;
                 r.lambda = j.pub.lambda;
                 r.tau = (T)pow((double)j.pub.L2, -0.5);   // TGV_core.c:54-55
@@ -975,7 +988,15 @@ private:
             case CILREG_FGP: {
                 auto* jp = (const FgpJob<T, S>*)t.dev;
                 if constexpr (DIM == 3) {
-                    if (w) {
+                    if (k.width == 1) {
+                        if (w) {
+                            if (iso) hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                            else hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        } else {
+                            if (iso) hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                            else hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        }
+                    } else if (w) {
                         if (iso) hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                         else hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                     } else {
@@ -1005,8 +1026,14 @@ private:
             default: {
                 auto* jp = (const TgvJob<T, S>*)t.dev;
                 if constexpr (DIM == 3) {
-                    if (w) hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (k.width == 1) {
+                        if (w) hipLaunchKernelGGL((tgv_group_kernel<3, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        else hipLaunchKernelGGL((tgv_group_kernel<3, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    } else if (w) {
+                        hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    } else {
+                        hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    }
                 } else {
                     if (w) hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                     else hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
@@ -1041,7 +1068,7 @@ private:
             const Table& t = *groups[gi].second;
             hipStream_t st = side_[gi];
             // 3D FGP and TGV groups advance two levels per launch (l0 is even)
-            const int width = ((k.method == CILREG_FGP || k.method == CILREG_TGV) && k.dim == 3) ? 2 : 1;
+            const int width = k.width;
             for (int lv = l0; lv < l1; lv += width) {
                 int nact = 0;
                 while (nact < t.njobs && t.ends[nact] > lv) ++nact;
@@ -1084,7 +1111,7 @@ private:
                 case CILREG_FGP: j.result = (const float*)j.fld[j.nbuf * d + (last & 1)]; break;
                 case CILREG_PD: j.result = stopped ? (const float*)j.fld[2 + 2 * d] : (const float*)j.fld[iters & 1]; break;
                 default:
-                    if (d == 3) {   // pass q = last / 2 wrote set (q + 1) & 1; Ualt on a stop at its first iteration
+                    if (j.width == 2) {   // pass q = last / 2 wrote set (q + 1) & 1; Ualt on a stop at its first iteration
                         const bool alt = stopped && (last & 1) == 0 && last + 1 < iters;
                         const int per = 2 * d + 6;
                         j.result = alt ? (const float*)j.fld[2 + 2 * per] : (const float*)j.fld[((last >> 1) + 1) & 1];

@@ -6,6 +6,7 @@
 // solver of its method and downloads the output; the *_dev extensions skip the
 // host copies.
 #include "solvers.hpp"
+#include "tune.hpp"
 #include "../../include/cilregcuda.h"
 #include <string.h>
 
@@ -156,9 +157,9 @@ int TGV_GPU_main(float* U0, float* U, float* infovector, float lambda, float alp
 // ---------------------------------------------------------------------------
 // device-resident extensions
 // ---------------------------------------------------------------------------
-#define DEV_PROLOGUE(NX, NY, NZ)                                 \
+#define DEV_PROLOGUE(NX, NY, NZ, METHOD, ISO)                    \
     if (!dims_ok(NX, NY, NZ)) return -1;                         \
-    const Opts o = resolve_opts(opts);                           \
+    const Opts o = tuned_opts(METHOD, NX, NY, (NZ) <= 1 ? 1 : (NZ), ISO, resolve_opts(opts)); \
     hipStream_t s = pick_stream(stream);                         \
     if (!s) { set_error("no stream"); return -1; }               \
     int done = 0;                                                \
@@ -175,14 +176,14 @@ int TGV_GPU_main(float* U0, float* U, float* infovector, float lambda, float alp
 int cilreg_rof_dev(const float* d_input, float* d_output, float* infovector, float lambdaPar, int iter, float tau,
                    float epsil, int dimX, int dimY, int dimZ, const cilreg_opts* opts, void* stream) {
     if (dimZ == 0) dimZ = 1;
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_ROF, 1)
     rc = rof_solve<float, float>(d_input, d_output, lambdaPar, iter, tau, epsil, dimX, dimY, dimZ, o, s, &done, &re);
     DEV_EPILOGUE()
 }
 
 int cilreg_fgp_dev(const float* d_input, float* d_output, float* infovector, float lambdaPar, int iter, float epsil,
                    int methodTV, int nonneg, int dimX, int dimY, int dimZ, const cilreg_opts* opts, void* stream) {
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_FGP, methodTV == 0)
     const int nz = dimZ <= 1 ? 1 : dimZ;
     if (o.precision == CILREG_BF16)
         rc = fgp_solve<float, bf16_t>(d_input, d_output, lambdaPar, iter, epsil, methodTV, nonneg, dimX, dimY, nz, o,
@@ -199,7 +200,7 @@ int cilreg_fgp_dev(const float* d_input, float* d_output, float* infovector, flo
 int cilreg_pd_dev(const float* d_input, float* d_output, float* infovector, float lambdaPar, int iter, float epsil,
                   float lipschitz_const, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
                   const cilreg_opts* opts, void* stream) {
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_PD, methodTV == 0)
     const int nz = dimZ <= 1 ? 1 : dimZ;
     if (o.precision == CILREG_BF16)
         rc = pd_solve<float, bf16_t>(d_input, d_output, lambdaPar, iter, epsil, lipschitz_const, methodTV, nonneg,
@@ -215,7 +216,7 @@ int cilreg_pd_dev(const float* d_input, float* d_output, float* infovector, floa
 
 int cilreg_sb_dev(const float* d_input, float* d_output, float* infovector, float mu, int iter, float epsil,
                   int methodTV, int dimX, int dimY, int dimZ, const cilreg_opts* opts, void* stream) {
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
     const int nz = dimZ <= 1 ? 1 : dimZ;
     rc = sb_solve<float, float>(d_input, d_output, mu, iter, epsil, methodTV, dimX, dimY, nz, o, s, &done, &re);
     DEV_EPILOGUE()
@@ -225,7 +226,7 @@ int cilreg_tgv_dev(const float* d_input, float* d_output, float* infovector, flo
                    int iter, float L2, float epsil, int dimX, int dimY, int dimZ, const cilreg_opts* opts,
                    void* stream) {
     if (dimZ == 0) dimZ = 1;
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_TGV, 1)
     if (o.precision == CILREG_BF16)
         rc = tgv_solve<float, bf16_t>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o,
                                       s, &done, &re);
@@ -241,7 +242,7 @@ int cilreg_tgv_dev(const float* d_input, float* d_output, float* infovector, flo
 int cilreg_fgp_dev_f64(const double* d_input, double* d_output, double* infovector, double lambdaPar, int iter,
                        double epsil, int methodTV, int nonneg, int dimX, int dimY, int dimZ, const cilreg_opts* opts,
                        void* stream) {
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
     const int nz = dimZ <= 1 ? 1 : dimZ;
     rc = fgp_solve<double, double>(d_input, d_output, lambdaPar, iter, epsil, methodTV, nonneg, dimX, dimY, nz, o, s,
                                    &done, &re);
@@ -252,7 +253,7 @@ int cilreg_tgv_dev_f64(const double* d_input, double* d_output, double* infovect
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts* opts, void* stream) {
     if (dimZ == 0) dimZ = 1;
-    DEV_PROLOGUE(dimX, dimY, dimZ)
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
     rc = tgv_solve<double, double>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o, s,
                                    &done, &re);
     DEV_EPILOGUE()

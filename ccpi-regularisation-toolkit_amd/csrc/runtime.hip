@@ -269,6 +269,7 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.wide_index = o->wide_index;
         r.stream_launches = o->stream_launches;
         r.depth = o->temporal_depth;
+        r.autotune = o->autotune;
     }
     return r;
 }

@@ -1,0 +1,177 @@
+// tune.hip -- the autotuner: chooses the tile height, the z-chunk depth and
+// the temporal depth (iterations per pass) of a solve by timing them.
+//
+// For a (method, dual storage, TV type, volume extent) it runs a short
+// fixed-count solve (four iterations) of a synthetic volume of that extent
+// for each candidate -- tile height 8 or 4, z chunk planned by occupancy or
+// 8 / 16 / 32 planes, one or two iterations per pass where the method has
+// both -- takes the per-iteration kernel time from HIP events (the solves'
+// own profiling), and keeps the fastest.  Results are cached per process,
+// keyed by the extent rounded up to 64 x 32 x 32, so a batch or a server pays
+// for a size class once.  Every candidate computes the same values bit for bit (the results
+// do not depend on the tiling or the pass width), so tuning never changes an
+// answer -- only, with a tolerance, the order of the FP64 residual sums.
+#include "solvers.hpp"
+#include "tune.hpp"
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace cilreg {
+
+namespace {
+
+struct Key {
+    int method, precision, iso, fixed_ty, nx, ny, nz;
+    bool operator<(const Key& o) const {
+        return std::tie(method, precision, iso, fixed_ty, nx, ny, nz) <
+               std::tie(o.method, o.precision, o.iso, o.fixed_ty, o.nx, o.ny, o.nz);
+    }
+};
+std::mutex g_mu;
+std::map<Key, TuneChoice> g_cache;
+
+// deterministic pseudo-random volume in [0, 1) (hash of the index)
+__global__ __launch_bounds__(256) void probe_fill_kernel(float* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        uint32_t h = (uint32_t)i * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        __builtin_nontemporal_store((float)(h >> 8) * (1.0f / 16777216.0f), &p[i]);
+    }
+}
+
+template <typename S>
+int run_method(int method, const float* in, float* out, int iso, int nx, int ny, int nz, int iters, const Opts& o,
+               hipStream_t s) {
+    int done = 0;
+    float re = 0.f;
+    switch (method) {
+        case CILREG_ROF: return rof_solve<float, float>(in, out, 0.02f, iters, 0.001f, 0.0f, nx, ny, nz, o, s, &done, &re);
+        case CILREG_FGP:
+            return fgp_solve<float, S>(in, out, 0.02f, iters, 0.0f, iso ? 0 : 1, 0, nx, ny, nz, o, s, &done, &re);
+        case CILREG_PD:
+            return pd_solve<float, S>(in, out, 0.02f, iters, 0.0f, 12.0f, iso ? 0 : 1, 0, nx, ny, nz, o, s, &done, &re);
+        default:
+            return tgv_solve<float, S>(in, out, 0.02f, 1.0f, 2.0f, iters, 12.0f, 0.0f, nx, ny, nz, o, s, &done, &re);
+    }
+}
+
+}  // namespace
+
+int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool fixed_ty, TuneChoice* out) {
+    *out = TuneChoice{};
+    if (method < CILREG_ROF || method > CILREG_TGV) {
+        set_error("autotune: unknown method %d", method);
+        return -1;
+    }
+    if (nz <= 1) {   // 2D: one plane per workgroup, 64 x 4 tiles, one iteration per pass
+        *out = TuneChoice{4, 1, 1, 0.0};
+        return 0;
+    }
+    const Key key{method, base.precision, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX, (ny + 31) / 32 * 32,
+                  (nz + 31) / 32 * 32};
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) {
+            *out = it->second;
+            return 0;
+        }
+    }
+    // Grouped execution shares the device with other jobs, so there the
+    // choice that matters is the cost per voxel with the device full: the
+    // probe then stacks copies of the extent along z up to ~32 M voxels.
+    int nzp = nz;
+    if (fixed_ty) {
+        const size_t per = (size_t)nx * ny * nz, target = (size_t)32 << 20;
+        size_t reps = per >= target ? 1 : (target + per - 1) / per;
+        while (reps > 1 && per * reps > ((size_t)64 << 20)) --reps;
+        nzp = (int)(nz * reps);
+    }
+    const size_t n = (size_t)nx * ny * nzp;
+    hipStream_t s = device_stream();
+    if (!s) { set_error("autotune: no stream"); return -1; }
+    Arena ar;
+    const size_t oi = ar.reserve(n * sizeof(float)), oo = ar.reserve(n * sizeof(float));
+    if (ar.allocate() != 0) return -1;
+    float* in = (float*)ar.at(oi);
+    float* outv = (float*)ar.at(oo);
+    {
+        size_t blocks = (n + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(probe_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, in, n);
+    }
+    std::vector<int> depths = {1};
+    if (method == CILREG_FGP || method == CILREG_TGV) depths = {2, 1};
+    std::vector<int> tys = fixed_ty ? std::vector<int>{8} : std::vector<int>{8, 4};
+    std::vector<int> zcs;
+    if (!fixed_ty) zcs.push_back(0);   // the occupancy-planned depth
+    // grouped: deep chunks only -- shallow ones win alone by filling the
+    // device with more workgroups, which a full group does not need
+    for (int z : fixed_ty ? std::initializer_list<int>{16, 32} : std::initializer_list<int>{8, 16, 32})
+        if (z <= nz) zcs.push_back(z);
+    if (zcs.empty()) zcs.push_back(nz);
+    TuneChoice best{8, fixed_ty ? zcs.front() : 0, depths.front(), 1e300};
+    const int iters = fixed_ty ? 6 : 4;
+    for (int d : depths) {
+        for (int ty : tys) {
+            for (int zc : zcs) {
+                Opts o = base;
+                o.tile_y = ty;
+                o.zchunk = zc;
+                o.depth = d;
+                o.profile = 1;
+                o.autotune = 0;
+                int rc;
+                if (base.precision == CILREG_BF16) rc = run_method<bf16_t>(method, in, outv, iso, nx, ny, nzp, iters, o, s);
+                else if (base.precision == CILREG_FP16)
+                    rc = run_method<_Float16>(method, in, outv, iso, nx, ny, nzp, iters, o, s);
+                else rc = run_method<float>(method, in, outv, iso, nx, ny, nzp, iters, o, s);
+                if (rc != 0) return -1;
+                const ProfileStats& ps = thread_profile();
+                if (ps.count > 0 && ps.mean_ms < best.ms_per_iter) best = TuneChoice{ty, zc, d, ps.mean_ms};
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = best;
+    *out = best;
+    return 0;
+}
+
+Opts tuned_opts(int method, int nx, int ny, int nz, int iso, const Opts& o) {
+    if (!o.autotune || o.tile_y != 0 || o.zchunk != 0 || o.depth != 0 || nz <= 1) return o;
+    if (method < CILREG_ROF || method > CILREG_TGV) return o;   // methods without tuning candidates
+    TuneChoice c;
+    if (autotune(method, nx, ny, nz, iso, o, false, &c) != 0) return o;   // keep the defaults
+    Opts r = o;
+    r.tile_y = c.tile_y;
+    r.zchunk = c.zchunk;
+    r.depth = c.depth;
+    return r;
+}
+
+}  // namespace cilreg
+
+using namespace cilreg;
+
+extern "C" CILREG_API int cilreg_autotune(int method, int dimX, int dimY, int dimZ, int methodTV,
+                                          const cilreg_opts* base, cilreg_opts* best, double* ms_per_iter) {
+    if (dimX <= 0 || dimY <= 0 || dimZ < 0) { set_error("invalid dimensions"); return -1; }
+    const Opts o = resolve_opts(base);
+    TuneChoice c;
+    if (autotune(method, dimX, dimY, dimZ <= 1 ? 1 : dimZ, methodTV == 0, o, false, &c) != 0) return -1;
+    if (best) {
+        if (base) *best = *base;
+        else *best = cilreg_opts{};
+        best->tile_y = c.tile_y;
+        best->zchunk = c.zchunk;
+        best->temporal_depth = c.depth;
+        best->autotune = 0;
+    }
+    if (ms_per_iter) *ms_per_iter = c.ms_per_iter;
+    return 0;
+}

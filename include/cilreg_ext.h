@@ -43,7 +43,8 @@ typedef struct cilreg_opts {
     int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
     int wide_index;      /* 1: force 64-bit element indexing (chosen automatically for fields >= 2 GiB) */
     int stream_launches; /* batch: 1 = launch directly instead of capturing one hipGraph */
-    int temporal_depth;  /* iterations fused per pass where supported (FGP 3D): 0 = auto (2), 1 = off */
+    int temporal_depth;  /* iterations fused per pass where supported (FGP, TGV 3D): 0 = auto (2), 1 = off */
+    int autotune;        /* 1: tile_y / zchunk / temporal_depth left at 0 are chosen by cilreg_autotune */
 } cilreg_opts;
 
 /* Methods of a batch or served job. */
@@ -118,6 +119,17 @@ CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, doubl
  * gbps[0] copy (1 read + 1 write), gbps[1] read, gbps[2] write, gbps[3]
  * 2 reads + 1 write.  Bytes moved / kernel time, in GB/s. */
 CILREG_API int cilreg_stream_bench(size_t bytes, int reps, double *gbps);
+
+/* The autotuner: times short fixed-count solves of a synthetic volume of the
+ * given extent for each candidate tiling (tile_y 8/4, zchunk planned or
+ * 8/16/32) and temporal depth (1/2 where the method has both) with base's dual
+ * storage, and writes the fastest into *best (base's other fields copied);
+ * *ms_per_iter receives its kernel time per iteration.  Cached per process
+ * and size class.  method: CILREG_ROF / _FGP / _PD / _TGV; methodTV as for
+ * FGP / PD.  The choice never changes results (tiling and depth are
+ * bit-neutral). */
+CILREG_API int cilreg_autotune(int method, int dimX, int dimY, int dimZ, int methodTV, const cilreg_opts *base,
+                               cilreg_opts *best, double *ms_per_iter);
 
 /* Runs every job of the batch to completion with grouped kernels: one launch
  * per iteration level and method group covers all volumes of that group, so
