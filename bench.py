@@ -212,7 +212,7 @@ def batched_jobs(seed=4):
 
 
 def batched_section(D, B, torch):
-    """Plain (one solve after another) vs one batched hipGraph launch, FP32 and BF16 duals."""
+    """Plain (one solve after another) vs one autotuned batched hipGraph launch, FP32 and BF16 / FP16 duals."""
     jobs = batched_jobs()
     xs = [B.phantom(shape, *ph) for shape, ph, _ in jobs]
     specs = [dict(sp, input=x) for (_, _, sp), x in zip(jobs, xs)]
@@ -241,8 +241,8 @@ def batched_section(D, B, torch):
     # size class; tuning happens here, outside the timed runs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    B.batch_run(short, autotune=True)
-    B.batch_run(short, precision="bf16", autotune=True)
+    for prec in ("fp32", "bf16", "fp16"):
+        B.batch_run(short, precision=prec, autotune=True)
     t_tune = time.perf_counter() - t0
     log("bench: batched section, plain FP32")
     ref, t_plain = timed(lambda: plain(specs, "fp32"))
@@ -250,17 +250,22 @@ def batched_section(D, B, torch):
     got32, t_b32 = timed(lambda: B.batch_run(specs, autotune=True))
     identical = all(bool(torch.equal(a, b)) for a, (b, _) in zip(ref, got32))
     del got32
-    log("bench: batched section, one graph BF16 duals")
-    got16, t_b16 = timed(lambda: B.batch_run(specs, precision="bf16", autotune=True))
-    # BF16 error against FP64 solves of the first four FGP volumes (ROF has no dual field)
-    errs = []
-    for sp, (y, _) in list(zip(specs, got16))[:8]:
-        if sp["method"] != "fgp":
-            continue
-        r64, _ = D.fgp_f64(sp["input"].double(), sp["lambda"], sp["iters"], 0.0, 0, 0)
-        errs.append(float(torch.linalg.vector_norm(y.double() - r64) / torch.linalg.vector_norm(r64)))
-        del r64
-    del got16, ref
+    out16 = {}
+    for prec in ("bf16", "fp16"):
+        log(f"bench: batched section, one graph {prec.upper()} duals")
+        got16, t16 = timed(lambda: B.batch_run(specs, precision=prec, autotune=True))
+        # error against FP64 solves of the first four FGP volumes (ROF has no dual field)
+        errs = []
+        for sp, (y, _) in list(zip(specs, got16))[:8]:
+            if sp["method"] != "fgp":
+                continue
+            r64, _ = D.fgp_f64(sp["input"].double(), sp["lambda"], sp["iters"], 0.0, 0, 0)
+            errs.append(float(torch.linalg.vector_norm(y.double() - r64) / torch.linalg.vector_norm(r64)))
+            del r64
+        del got16
+        out16[prec] = {"seconds": round(t16, 4), "voxel_iters_per_s": total_vi / t16, "speedup_vs_plain": t_plain / t16,
+                       "fgp_rel_l2_vs_fp64_max_of_4": max(errs) if errs else None}
+    del ref
     return {
         "workload": "64 volumes, extents U{64..320}^3, half FGP-TV 300 iterations, half ROF-TV 1000 iterations",
         "voxel_iters": total_vi,
@@ -268,9 +273,8 @@ def batched_section(D, B, torch):
         "plain_fp32": {"seconds": round(t_plain, 4), "voxel_iters_per_s": total_vi / t_plain},
         "batched_graph_fp32": {"seconds": round(t_b32, 4), "voxel_iters_per_s": total_vi / t_b32,
                                "speedup_vs_plain": t_plain / t_b32, "bit_identical_to_plain": identical},
-        "batched_graph_bf16": {"seconds": round(t_b16, 4), "voxel_iters_per_s": total_vi / t_b16,
-                               "speedup_vs_plain": t_plain / t_b16,
-                               "fgp_rel_l2_vs_fp64_max_of_4": max(errs) if errs else None},
+        "batched_graph_bf16": out16["bf16"],
+        "batched_graph_fp16": out16["fp16"],
     }
 
 
@@ -312,7 +316,7 @@ def served_section(D, B, torch, window):
     # warm-up of code objects and workspaces
     with B.Server(0, hbm_budget=200e9) as srv:
         B.serve_stream(srv, [(0.0, dict(sp, iters=2)) for _, sp in reqs[:8]])
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "bf16", "fp16"):
         log(f"bench: served stream {prec}")
         with B.Server(0, hbm_budget=200e9, precision=prec) as srv:
             r = B.serve_stream(srv, reqs, levels=8)
@@ -356,7 +360,7 @@ def served_section(D, B, torch, window):
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     out["plain_fp32"] = {"requests": len(reqs), "seconds": round(tp, 4), "voxel_iters_per_s": vi / tp}
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "bf16", "fp16"):
         out[f"served_{prec}"]["vs_plain_voxel_iters"] = (out[f"served_{prec}"]["voxel_iters_per_s"]
                                                           / out["plain_fp32"]["voxel_iters_per_s"])
     return out
