@@ -66,7 +66,9 @@ template <typename T, typename S> struct PdJob {
     const T* f;
     T* U[2];
     S* P[2][3];
-    T* Ubrk;
+    T* Ubrk;            // U of a checked iteration (the first of a two-iteration pass)
+    T* Ubrk1;           // 3D, two-iteration passes: U of the second iteration when checked
+    double* partials1;  // residual sums of the second iteration of a pass
     T sigma, tau, lt;
     int nonneg;
 };
@@ -325,6 +327,65 @@ __global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* 
     tgv_body<DIM, TY, T, S, I>(a, tile_of_logical(a.g, TY, local));
 }
 
+// 3D PD groups with two-iteration passes: pass q = k / 2 of a job reads
+// state set q & 1 and writes the other; a job's last iteration of an odd
+// count runs the one-iteration body on the same sets.
+template <int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd2_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+    const PdJob<T, S>* j;
+    int local;
+    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    if (k < 0) return;
+    const int cur = (k >> 1) & 1, nxt = cur ^ 1;
+    const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
+    if (k + 1 < j->h.iters) {
+        Pd2Args<T, S> a;
+        a.f = uni(j->f);
+        a.U = uni(j->U[cur]);
+        a.Uout = uni(j->U[nxt]);
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.P[c] = uni(j->P[cur][c]);
+            a.Pout[c] = uni(j->P[nxt][c]);
+        }
+        a.Ubrk0 = uni(j->Ubrk);
+        a.Ubrk1 = uni(j->Ubrk1);
+        a.sigma = unf(j->sigma);
+        a.tau = unf(j->tau);
+        a.lt = unf(j->lt);
+        a.theta = 1;
+        a.nonneg = j->nonneg;
+        a.check0 = job_checks(j->h, k);
+        a.check1 = job_checks(j->h, k + 1);
+        a.partials0 = uni(j->h.partials);
+        a.partials1 = uni(j->partials1);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        pd2_body<TY, T, S, ISO, I>(a, tc);
+    } else {
+        PdArgs<T, S> a;
+        a.f = uni(j->f);
+        a.U = uni(j->U[cur]);
+        a.Uout = uni(j->U[nxt]);
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.P[c] = uni(j->P[cur][c]);
+            a.Pout[c] = uni(j->P[nxt][c]);
+        }
+        a.Ubrk = uni(j->Ubrk);
+        a.sigma = unf(j->sigma);
+        a.tau = unf(j->tau);
+        a.lt = unf(j->lt);
+        a.theta = 1;
+        a.nonneg = j->nonneg;
+        a.check = job_checks(j->h, k);
+        a.partials = uni(j->h.partials);
+        a.ctrl = uni(j->h.ctrl);
+        a.g = uni_geom(j->h.g);
+        pd_body<3, TY, T, S, ISO, I>(a, tc);
+    }
+}
+
 // 3D TGV groups advance two levels per launch: pass q = k / 2 of a job reads
 // state set q & 1 and writes the other; a job's last iteration of an odd
 // count runs the one-iteration body on the same sets.
@@ -562,7 +623,7 @@ public:
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
         j->ty = j->dim == 3 ? 8 : 4;
-        const bool two = (j->pub.method == CILREG_FGP || j->pub.method == CILREG_TGV) && j->dim == 3;
+        const bool two = j->pub.method != CILREG_ROF && j->dim == 3;
         int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
         j->width = two && opts_.depth != 1 ? 2 : 1;
         if (opts_.autotune && j->dim == 3 && opts_.zchunk == 0 && opts_.depth == 0) {
@@ -708,7 +769,12 @@ private:
         switch (j.pub.method) {
             case CILREG_ROF: res(nT); res(nT); break;                                   // U x2
             case CILREG_FGP: for (int i = 0; i < j.nbuf * d; ++i) res(nS); res(nT); res(nT); break;   // P sets, D x2
-            case CILREG_PD: res(nT); res(nT); for (int i = 0; i < 2 * d; ++i) res(nS); res(nT); break;  // U x2, P x2, Ubrk
+            case CILREG_PD:   // U x2, P x2, Ubrk (+ Ubrk1 with two-iteration passes and a tolerance)
+                res(nT); res(nT);
+                for (int i = 0; i < 2 * d; ++i) res(nS);
+                res(nT);
+                if (j.width == 2 && j.pub.epsil != 0.0f) res(nT);
+                break;
             default:                                                                       // U x2, (V, P, Q) x2
                 res(nT); res(nT);
                 for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
@@ -890,6 +956,8 @@ private:
                 for (int b = 0; b < 2; ++b)
                     for (int c = 0; c < 3; ++c) r.P[b][c] = (S*)ch(2 + b * d, c, d);
                 r.Ubrk = (T*)j.fld[2 + 2 * d];
+                r.Ubrk1 = (j.width == 2 && j.pub.epsil != 0.0f) ? (T*)j.fld[3 + 2 * d] : nullptr;
+                r.partials1 = j.partials + 2 * (size_t)j.g.nblocks;
                 const T tau = j.pub.lambda * (T)0.1;   // PD_TV_core.c:50-53
                 r.tau = tau;
                 r.sigma = (T)(1.0 / (double)(j.pub.lipschitz * tau));
@@ -1014,6 +1082,18 @@ private:
             }
             case CILREG_PD: {
                 auto* jp = (const PdJob<T, S>*)t.dev;
+                if constexpr (DIM == 3) {
+                    if (k.width == 2) {
+                        if (w) {
+                            if (iso) hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                            else hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        } else {
+                            if (iso) hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                            else hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        }
+                        break;
+                    }
+                }
                 if (w) {
                     if (iso) hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                     else hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
@@ -1109,7 +1189,15 @@ private:
             switch (j.pub.method) {
                 case CILREG_ROF: j.result = (const float*)j.fld[(last + 1) & 1]; break;
                 case CILREG_FGP: j.result = (const float*)j.fld[j.nbuf * d + (last & 1)]; break;
-                case CILREG_PD: j.result = stopped ? (const float*)j.fld[2 + 2 * d] : (const float*)j.fld[iters & 1]; break;
+                case CILREG_PD:
+                    if (j.width == 2) {   // pass q = last / 2 wrote set (q + 1) & 1
+                        const bool second = (last & 1) == 1;   // a stop in the second iteration of a pass
+                        j.result = stopped ? (const float*)j.fld[second ? 3 + 2 * d : 2 + 2 * d]
+                                           : (const float*)j.fld[((last >> 1) + 1) & 1];
+                    } else {
+                        j.result = stopped ? (const float*)j.fld[2 + 2 * d] : (const float*)j.fld[iters & 1];
+                    }
+                    break;
                 default:
                     if (j.width == 2) {   // pass q = last / 2 wrote set (q + 1) & 1; Ualt on a stop at its first iteration
                         const bool alt = stopped && (last & 1) == 0 && last + 1 < iters;

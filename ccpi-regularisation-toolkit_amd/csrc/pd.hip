@@ -20,6 +20,7 @@
 // z-1 from a register.
 #include "solvers.hpp"
 #include "pd_kernels.hpp"
+#include <vector>
 
 namespace cilreg {
 
@@ -40,6 +41,32 @@ static void launch_pd(const PdArgs<T, S>& a, bool iso, bool small, hipStream_t s
     }
 }
 
+template <int TY, typename T, typename S>
+static const void* pd2_kernel_ptr(bool iso, bool small) {
+    if (small)
+        return iso ? (const void*)pd2_iter_kernel<TY, T, S, true, uint32_t>
+                   : (const void*)pd2_iter_kernel<TY, T, S, false, uint32_t>;
+    return iso ? (const void*)pd2_iter_kernel<TY, T, S, true, uint64_t>
+               : (const void*)pd2_iter_kernel<TY, T, S, false, uint64_t>;
+}
+
+template <int TY, typename T, typename S>
+static void launch_pd2(const Pd2Args<T, S>& a, bool iso, bool small, hipStream_t s) {
+    dim3 block(TX, TY), grid(a.g.nblocks);
+    if (small) {
+        if (iso) hipLaunchKernelGGL((pd2_iter_kernel<TY, T, S, true, uint32_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((pd2_iter_kernel<TY, T, S, false, uint32_t>), grid, block, 0, s, a);
+    } else {
+        if (iso) hipLaunchKernelGGL((pd2_iter_kernel<TY, T, S, true, uint64_t>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((pd2_iter_kernel<TY, T, S, false, uint64_t>), grid, block, 0, s, a);
+    }
+}
+
+// Device-resident PD solve.  In 3D, iterations run two per pass
+// (pd2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
+// 2D iteration, runs on the one-iteration kernel.  Each pass reads state set
+// `cur` (u_bar, P) and writes set cur ^ 1; checked iterations leave the
+// un-extrapolated U in Ubrk0 (first iteration of a pass) or Ubrk1 (second).
 template <typename T, typename S>
 int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, int method, int nonneg, int nx, int ny,
              int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re) {
@@ -56,9 +83,13 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
+    else if (width == 2)
+        g = ty == 8 ? plan_geom(pd2_kernel_ptr<8, T, S>(iso, small), TX * 8, nx, ny, nz, 8, opts.zchunk, 2.5)
+                    : plan_geom(pd2_kernel_ptr<4, T, S>(iso, small), TX * 4, nx, ny, nz, 4, opts.zchunk, 2.5);
     else if (ty == 8)
         g = plan_geom(small ? pd_kernel_ptr<3, 8, T, S, uint32_t>(iso) : pd_kernel_ptr<3, 8, T, S, uint64_t>(iso),
                       TX * 8, nx, ny, nz, 8, opts.zchunk);
@@ -66,61 +97,107 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
         g = plan_geom(small ? pd_kernel_ptr<3, 4, T, S, uint32_t>(iso) : pd_kernel_ptr<3, 4, T, S, uint64_t>(iso),
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
 
+    const bool use_tol = epsil != (T)0;
     Arena ar;
-    size_t oU[2], oP[2][3], oB, oCtrl, oPart;
+    size_t oU[2], oP[2][3], oB[2] = {0, 0}, oCtrl, oPart;
     for (int b = 0; b < 2; ++b) oU[b] = ar.reserve(n * sizeof(T));
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
-    oB = ar.reserve(n * sizeof(T));
+    oB[0] = ar.reserve(n * sizeof(T));
+    if (width == 2 && use_tol) oB[1] = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
-    oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
+    oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
     launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
     for (int c = 0; c < dim; ++c) launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
-    double* partials = (double*)ar.at(oPart);
+    double* partials0 = (double*)ar.at(oPart);
+    double* partials1 = partials0 + 2 * (size_t)g.nblocks;
 
     // step sizes in the working precision (PD_TV_core.c:50-53)
     const T tau = lambda * (T)0.1;
     const T sigma = (T)(1.0 / (double)(lipschitz * tau));
     const T lt = tau / lambda;
-    auto step = [&](int k, bool check) {
-        PdArgs<T, S> a;
-        const int cur = k & 1, nxt = cur ^ 1;
-        a.f = d_f;
-        a.U = (const T*)ar.at(oU[cur]);
-        a.Uout = (T*)ar.at(oU[nxt]);
-        for (int c = 0; c < 3; ++c) {
-            const int cc = c < dim ? c : 0;
-            a.P[c] = (const S*)ar.at(oP[cur][cc]);
-            a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
-        }
-        a.Ubrk = (T*)ar.at(oB);
-        a.sigma = sigma;
-        a.tau = tau;
-        a.lt = lt;
-        a.theta = 1;
-        a.nonneg = nonneg != 0;
-        a.check = check;
-        a.partials = partials;
-        a.ctrl = ctrl;
-        a.g = g;
-        if (dim == 3) {
-            if (ty == 8) launch_pd<3, 8>(a, iso, small, s);
-            else launch_pd<3, 4>(a, iso, small, s);
+    int cur = 0;
+    std::vector<int> pass_k, pass_n, pass_out;
+    auto step = [&](int k, int nk, bool c0, bool c1) {
+        const int nxt = cur ^ 1;
+        pass_k.push_back(k);
+        pass_n.push_back(nk);
+        pass_out.push_back(nxt);
+        if (nk == 2) {
+            Pd2Args<T, S> a;
+            a.f = d_f;
+            a.U = (const T*)ar.at(oU[cur]);
+            a.Uout = (T*)ar.at(oU[nxt]);
+            for (int c = 0; c < 3; ++c) {
+                a.P[c] = (const S*)ar.at(oP[cur][c]);
+                a.Pout[c] = (S*)ar.at(oP[nxt][c]);
+            }
+            a.Ubrk0 = (T*)ar.at(oB[0]);
+            a.Ubrk1 = use_tol ? (T*)ar.at(oB[1]) : nullptr;
+            a.sigma = sigma;
+            a.tau = tau;
+            a.lt = lt;
+            a.theta = 1;
+            a.nonneg = nonneg != 0;
+            a.check0 = c0;
+            a.check1 = c1;
+            a.partials0 = partials0;
+            a.partials1 = partials1;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (ty == 8) launch_pd2<8>(a, iso, small, s);
+            else launch_pd2<4>(a, iso, small, s);
         } else {
-            launch_pd<2, 4>(a, iso, small, s);
+            PdArgs<T, S> a;
+            a.f = d_f;
+            a.U = (const T*)ar.at(oU[cur]);
+            a.Uout = (T*)ar.at(oU[nxt]);
+            for (int c = 0; c < 3; ++c) {
+                const int cc = c < dim ? c : 0;
+                a.P[c] = (const S*)ar.at(oP[cur][cc]);
+                a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
+            }
+            a.Ubrk = (T*)ar.at(oB[0]);
+            a.sigma = sigma;
+            a.tau = tau;
+            a.lt = lt;
+            a.theta = 1;
+            a.nonneg = nonneg != 0;
+            a.check = c0;
+            a.partials = partials0;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (dim == 3) {
+                if (ty == 8) launch_pd<3, 8>(a, iso, small, s);
+                else launch_pd<3, 4>(a, iso, small, s);
+            } else {
+                launch_pd<2, 4>(a, iso, small, s);
+            }
         }
+        cur = nxt;
     };
     int done = 0;
-    if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
+    if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials0, partials1, g.nblocks, s, step, &done, re) != 0)
+        return -1;
     *iters_done = done;
-    const T* src = (done < iter) ? (const T*)ar.at(oB) : (const T*)ar.at(oU[iter & 1]);
+    // a stop returns U of the stop iteration (before extrapolation); else u_bar after the last pass
+    const void* src = nullptr;
+    if (done < iter) {
+        for (size_t q = 0; q < pass_k.size(); ++q)
+            if (done >= pass_k[q] && done < pass_k[q] + pass_n[q]) {
+                src = (pass_n[q] == 2 && done == pass_k[q] + 1) ? ar.at(oB[1]) : ar.at(oB[0]);
+                break;
+            }
+    } else {
+        src = ar.at(oU[pass_out.back()]);
+    }
+    if (!src) { set_error("PD: no pass covers iteration %d", done); return -1; }
     CILREG_CK(hipMemcpyAsync(d_out, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
-
 template int pd_solve<float, float>(const float*, float*, float, int, float, float, int, int, int, int, int,
                                     const Opts&, hipStream_t, int*, float*);
 template int pd_solve<float, bf16_t>(const float*, float*, float, int, float, float, int, int, int, int, int,

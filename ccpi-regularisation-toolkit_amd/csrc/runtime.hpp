@@ -130,7 +130,7 @@ struct Opts {
     int profile = 0;       // time each iteration with HIP events
     int wide_index = 0;    // force 64-bit element indexing
     int stream_launches = 0;  // batch: launch directly instead of one hipGraph
-    int depth = 0;         // iterations per pass where supported (FGP, TGV 3D: 2 by default; 1 = off)
+    int depth = 0;         // iterations per pass where supported (FGP, PD, TGV 3D: 2 by default; 1 = off)
     int autotune = 0;      // choose tile_y / zchunk / depth by timing (tune.hip) when none is given
 };
 

@@ -105,7 +105,7 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
         hipLaunchKernelGGL(probe_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, in, n);
     }
     std::vector<int> depths = {1};
-    if (method == CILREG_FGP || method == CILREG_TGV) depths = {2, 1};
+    if (method != CILREG_ROF) depths = {2, 1};
     std::vector<int> tys = fixed_ty ? std::vector<int>{8} : std::vector<int>{8, 4};
     std::vector<int> zcs;
     if (!fixed_ty) zcs.push_back(0);   // the occupancy-planned depth

@@ -43,7 +43,7 @@ typedef struct cilreg_opts {
     int profile;         /* 1: time every iteration's kernels with HIP events (see cilreg_last_profile) */
     int wide_index;      /* 1: force 64-bit element indexing (chosen automatically for fields >= 2 GiB) */
     int stream_launches; /* batch: 1 = launch directly instead of capturing one hipGraph */
-    int temporal_depth;  /* iterations fused per pass where supported (FGP, TGV 3D): 0 = auto (2), 1 = off */
+    int temporal_depth;  /* iterations fused per pass where supported (FGP, PD, TGV 3D): 0 = auto (2), 1 = off */
     int autotune;        /* 1: tile_y / zchunk / temporal_depth left at 0 are chosen by cilreg_autotune */
 } cilreg_opts;
 
@@ -122,7 +122,7 @@ CILREG_API int cilreg_stream_bench(size_t bytes, int reps, double *gbps);
 
 /* The autotuner: times short fixed-count solves of a synthetic volume of the
  * given extent for each candidate tiling (tile_y 8/4, zchunk planned or
- * 8/16/32) and temporal depth (1/2 where the method has both) with base's dual
+ * 8/16/32) and temporal depth (1/2 for FGP, PD, TGV in 3D) with base's dual
  * storage, and writes the fastest into *best (base's other fields copied);
  * *ms_per_iter receives its kernel time per iteration.  Cached per process
  * and size class.  method: CILREG_ROF / _FGP / _PD / _TGV; methodTV as for

@@ -121,3 +121,53 @@ def test_tgv_depth2_wide_index():
     x = _vol((12, 30, 66), 24)
     a, b = _tgv_pair(x, lam=0.03, alpha1=1.0, alpha0=2.0, iters=9, wide_index=True)
     _same(a, b, "TGV wide index")
+
+
+# ---------------------------------------------------------------------------
+# PD: the two-iteration pass (state u_bar, P), including the mirrored forward
+# difference at the last index of every axis and the un-extrapolated stop
+# output of either iteration of a pass.
+# ---------------------------------------------------------------------------
+def _pd_pair(x, **kw):
+    from ccpi.filters import device as D
+    a = D.pd(x, depth=1, **kw)
+    b = D.pd(x, depth=2, **kw)
+    return a, b
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("zchunk", [0, 1, 2, 3, 7])
+def test_pd_depth2_matches_depth1_tilings(shape, zchunk):
+    x = _vol(shape, 31)
+    for tile_y in (4, 8):
+        for iters in (1, 2, 7, 10):
+            a, b = _pd_pair(x, lam=0.03, iters=iters, L=12.0, zchunk=zchunk, tile_y=tile_y, nonneg=1)
+            _same(a, b, f"PD {shape} zchunk={zchunk} tile_y={tile_y} iters={iters}")
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["iso", "aniso"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_pd_depth2_matches_depth1_methods(method, precision):
+    x = _vol((24, 40, 72), 32)
+    for nonneg in (0, 1):
+        a, b = _pd_pair(x, lam=0.02, iters=25, L=8.0, method=method, nonneg=nonneg, precision=precision)
+        _same(a, b, f"PD method={method} nonneg={nonneg} {precision}")
+
+
+@pytest.mark.parametrize("check_every", [1, 2, 3, 5])
+def test_pd_depth2_matches_depth1_tolerance_stops(check_every):
+    x = _vol((16, 48, 80), 33)
+    parities = set()
+    for tol in (3e-2, 1e-2, 3e-3):
+        for zchunk in (0, 4):
+            a, b = _pd_pair(x, lam=0.03, iters=150, tol=tol, L=12.0, check_every=check_every, zchunk=zchunk)
+            _same(a, b, f"PD tol={tol} check_every={check_every} zchunk={zchunk}")
+            if int(a[1][0]) < 150:
+                parities.add(int(a[1][0]) % 2)
+    assert parities, "no tolerance stop exercised"
+
+
+def test_pd_depth2_wide_index():
+    x = _vol((12, 30, 66), 34)
+    a, b = _pd_pair(x, lam=0.03, iters=9, wide_index=True)
+    _same(a, b, "PD wide index")

@@ -20,7 +20,7 @@ def test_autotune_choice_is_a_candidate_and_cached(method):
     c = D.autotune(method, shape)
     assert c["tile_y"] in (4, 8)
     assert c["zchunk"] in (0, 8, 16)
-    assert c["depth"] in ((1, 2) if method in ("fgp", "tgv") else (1,))
+    assert c["depth"] in ((1, 2) if method != "rof" else (1,))
     assert c["ms_per_iter"] > 0
     t0 = time.perf_counter()
     again = D.autotune(method, shape)
