@@ -23,6 +23,27 @@ struct RofArgs {
 __device__ inline int mirror_p(int i, int n) { return i + 1 >= n ? (i - 1 >= 0 ? i - 1 : 0) : i + 1; }
 __device__ inline int mirror_m(int i, int n) { return i - 1 < 0 ? (i + 1 < n ? i + 1 : 0) : i - 1; }
 
+// The fluxes D1 (along y), D2 (along x) and, in 3D, D3 (along z) at a point
+// from u there (uc) and at its mirrored neighbours (ROF_TV_core.c:102-266).
+template <int DIM, typename T>
+__device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T ukm, T& d1, T& d2, T& d3) {
+    const T fi = uip - uc, bi = uc - uim, fj = ujp - uc, bj = uc - ujm;
+    const T mi = Mixed<T>::minmod_sq(fi, bi), mj = Mixed<T>::minmod_sq(fj, bj);
+    if (DIM == 3) {
+        const T fk = ukp - uc, bk = uc - ukm;
+        const T mk = Mixed<T>::minmod_sq(fk, bk);
+        // ROF_TV_core.c:131-137 (double sqrt), :194-200, :257-263
+        d1 = fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk);
+        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk);
+        d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi);
+    } else {
+        // ROF_TV_core.c:157-161, :220-224
+        d1 = fj / Mixed<T>::rof_sqrt_f(fj * fj + mi);
+        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj);
+        d3 = 0;
+    }
+}
+
 // The fluxes D1 (along y), D2 (along x) and, in 3D, D3 (along z) at
 // (px, py, z) from one load of the 7-point mirrored neighbourhood.
 template <int DIM, typename I, typename T>
@@ -40,21 +61,7 @@ __device__ inline void rof_fluxes(const RofArgs<T>& a, int px, int py, int z, T&
         ukp = ldx<T>(a.U, (I)mirror_p(z, g.nz) * sxy + (I)py * nx + (I)px);
         ukm = ldx<T>(a.U, (I)mirror_m(z, g.nz) * sxy + (I)py * nx + (I)px);
     }
-    const T fi = uip - uc, bi = uc - uim, fj = ujp - uc, bj = uc - ujm;
-    const T mi = Mixed<T>::minmod_sq(fi, bi), mj = Mixed<T>::minmod_sq(fj, bj);
-    if (DIM == 3) {
-        const T fk = ukp - uc, bk = uc - ukm;
-        const T mk = Mixed<T>::minmod_sq(fk, bk);
-        // ROF_TV_core.c:131-137 (double sqrt), :194-200, :257-263
-        d1 = fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk);
-        d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi);
-    } else {
-        // ROF_TV_core.c:157-161, :220-224
-        d1 = fj / Mixed<T>::rof_sqrt_f(fj * fj + mi);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj);
-        d3 = 0;
-    }
+    rof_flux_math<DIM, T>(uc, uip, uim, ujp, ujm, ukp, ukm, d1, d2, d3);
 }
 
 template <int DIM, int TY, typename T, typename I>
