@@ -96,6 +96,9 @@ _SIGS = {
     "cilreg_tgv_dev": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev_f64": [_vp, _vp, _vp, _d, _d, _d, _i, _d, _d, _i, _i, _i, _vp, _vp],
+    "cilreg_fgp_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
+    "cilreg_pd_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _i, _vp, _vp],
+    "cilreg_tgv_dev_ws": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _i, _vp, _vp],
 }
 
 
@@ -182,6 +185,82 @@ def tgv_f64(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
     out = _prep(x, out, torch.float64)
     return _run("cilreg_tgv_dev_f64", x, out, torch.float64, [lam, alpha1, alpha0, iters, L2, tol],
                 make_opts(**kw))
+
+
+# ---------------------------------------------------------------------------
+# warm start (include/cilreg_ext.h: cilreg_*_dev_ws)
+# ---------------------------------------------------------------------------
+STATE_CHANNELS = {"fgp": 6, "pd": 4, "tgv": 13}
+
+
+def new_state(method, x):
+    """Zeroed iterate state for `method` ('fgp', 'pd', 'tgv') on volumes like x
+    (FP32 channels of x's shape); for FGP also the host pair (t_k, next
+    multiplier) as a float32 array."""
+    import torch
+    st = torch.zeros((STATE_CHANNELS[method],) + tuple(x.shape), dtype=torch.float32, device=x.device)
+    if method == "fgp":
+        return st, np.array([1.0, 0.0], dtype=np.float32)
+    return st
+
+
+def _check_state(st, method, x):
+    import torch
+    if not (st.is_cuda and st.dtype == torch.float32 and st.is_contiguous()
+            and tuple(st.shape) == (STATE_CHANNELS[method],) + tuple(x.shape)):
+        raise ValueError(f"state must be a contiguous float32 CUDA tensor of shape "
+                         f"{(STATE_CHANNELS[method],) + tuple(x.shape)}")
+
+
+def fgp_warm(x, lam, iters, state, tstate, warm=True, tol=0.0, method=0, nonneg=0, out=None, **kw):
+    """FGP starting from (warm) and returning the iterate state in `state`
+    (P, P_prev) and `tstate` (t_k, next multiplier; updated in place)."""
+    import torch
+    out = _prep(x, out, torch.float32)
+    _check_state(state, "fgp", x)
+    ts = np.ascontiguousarray(tstate, dtype=np.float32)
+    info = np.zeros(2, dtype=np.float32)
+    f = _fn("cilreg_fgp_dev_ws")
+    with torch.cuda.device(x.device):
+        rc = f(x.data_ptr(), out.data_ptr(), info.ctypes.data, lam, iters, tol, method, nonneg, *_dims(x),
+               state.data_ptr(), ts.ctypes.data, int(bool(warm)), ctypes.byref(make_opts(**kw)),
+               ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"cilreg_fgp_dev_ws failed: {last_error()}")
+    tstate[...] = ts
+    return out, info
+
+
+def pd_warm(x, lam, iters, state, warm=True, tol=0.0, L=12.0, method=0, nonneg=0, out=None, **kw):
+    """PD starting from (warm) and returning the iterate state (u_bar, P) in `state`."""
+    import torch
+    out = _prep(x, out, torch.float32)
+    _check_state(state, "pd", x)
+    info = np.zeros(2, dtype=np.float32)
+    f = _fn("cilreg_pd_dev_ws")
+    with torch.cuda.device(x.device):
+        rc = f(x.data_ptr(), out.data_ptr(), info.ctypes.data, lam, iters, tol, L, method, nonneg, *_dims(x),
+               state.data_ptr(), int(bool(warm)), ctypes.byref(make_opts(**kw)),
+               ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"cilreg_pd_dev_ws failed: {last_error()}")
+    return out, info
+
+
+def tgv_warm(x, lam, alpha1, alpha0, iters, state, warm=True, L2=12.0, tol=0.0, out=None, **kw):
+    """TGV starting from (warm) and returning the iterate state (u_bar, V, P, Q) in `state`."""
+    import torch
+    out = _prep(x, out, torch.float32)
+    _check_state(state, "tgv", x)
+    info = np.zeros(2, dtype=np.float32)
+    f = _fn("cilreg_tgv_dev_ws")
+    with torch.cuda.device(x.device):
+        rc = f(x.data_ptr(), out.data_ptr(), info.ctypes.data, lam, alpha1, alpha0, iters, L2, tol, *_dims(x),
+               state.data_ptr(), int(bool(warm)), ctypes.byref(make_opts(**kw)),
+               ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"cilreg_tgv_dev_ws failed: {last_error()}")
+    return out, info
 
 
 def run(method, x, params, out=None, **kw):

@@ -259,4 +259,64 @@ int cilreg_tgv_dev_f64(const double* d_input, double* d_output, double* infovect
     DEV_EPILOGUE()
 }
 
+// ---------------------------------------------------------------------------
+// warm start: the iterate state in caller memory (FP32 channels of n values)
+// ---------------------------------------------------------------------------
+int cilreg_fgp_dev_ws(const float* d_input, float* d_output, float* infovector, float lambdaPar, int iter, float epsil,
+                      int methodTV, int nonneg, int dimX, int dimY, int dimZ, float* d_state, float* t_state, int warm,
+                      const cilreg_opts* opts, void* stream) {
+    if (!d_state || !t_state || iter < 1) { set_error("warm start needs d_state, t_state and iter >= 1"); return -1; }
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_FGP, methodTV == 0)
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    WarmState ws{d_state, t_state, warm != 0};
+    if (o.precision == CILREG_BF16)
+        rc = fgp_solve<float, bf16_t>(d_input, d_output, lambdaPar, iter, epsil, methodTV, nonneg, dimX, dimY, nz, o,
+                                      s, &done, &re, &ws);
+    else if (o.precision == CILREG_FP16)
+        rc = fgp_solve<float, _Float16>(d_input, d_output, lambdaPar, iter, epsil, methodTV, nonneg, dimX, dimY, nz,
+                                        o, s, &done, &re, &ws);
+    else
+        rc = fgp_solve<float, float>(d_input, d_output, lambdaPar, iter, epsil, methodTV, nonneg, dimX, dimY, nz, o, s,
+                                     &done, &re, &ws);
+    DEV_EPILOGUE()
+}
+
+int cilreg_pd_dev_ws(const float* d_input, float* d_output, float* infovector, float lambdaPar, int iter, float epsil,
+                     float lipschitz_const, int methodTV, int nonneg, int dimX, int dimY, int dimZ, float* d_state,
+                     int warm, const cilreg_opts* opts, void* stream) {
+    if (!d_state || iter < 1) { set_error("warm start needs d_state and iter >= 1"); return -1; }
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_PD, methodTV == 0)
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    WarmState ws{d_state, nullptr, warm != 0};
+    if (o.precision == CILREG_BF16)
+        rc = pd_solve<float, bf16_t>(d_input, d_output, lambdaPar, iter, epsil, lipschitz_const, methodTV, nonneg,
+                                     dimX, dimY, nz, o, s, &done, &re, &ws);
+    else if (o.precision == CILREG_FP16)
+        rc = pd_solve<float, _Float16>(d_input, d_output, lambdaPar, iter, epsil, lipschitz_const, methodTV, nonneg,
+                                       dimX, dimY, nz, o, s, &done, &re, &ws);
+    else
+        rc = pd_solve<float, float>(d_input, d_output, lambdaPar, iter, epsil, lipschitz_const, methodTV, nonneg,
+                                    dimX, dimY, nz, o, s, &done, &re, &ws);
+    DEV_EPILOGUE()
+}
+
+int cilreg_tgv_dev_ws(const float* d_input, float* d_output, float* infovector, float lambda, float alpha1,
+                      float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ, float* d_state,
+                      int warm, const cilreg_opts* opts, void* stream) {
+    if (!d_state || iter < 1) { set_error("warm start needs d_state and iter >= 1"); return -1; }
+    if (dimZ == 0) dimZ = 1;
+    DEV_PROLOGUE(dimX, dimY, dimZ, CILREG_TGV, 1)
+    WarmState ws{d_state, nullptr, warm != 0};
+    if (o.precision == CILREG_BF16)
+        rc = tgv_solve<float, bf16_t>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o,
+                                      s, &done, &re, &ws);
+    else if (o.precision == CILREG_FP16)
+        rc = tgv_solve<float, _Float16>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ,
+                                        o, s, &done, &re, &ws);
+    else
+        rc = tgv_solve<float, float>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o,
+                                     s, &done, &re, &ws);
+    DEV_EPILOGUE()
+}
+
 }  // extern "C"

@@ -73,7 +73,7 @@ static void launch_fgp2(const Fgp2Args<T, S>& a, bool iso, bool small, hipStream
 // P_{k+2} while reading P_k and P_{k-1}; 3 otherwise).
 template <typename T, typename S>
 int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, int nonneg, int nx, int ny, int nz,
-              const Opts& opts, hipStream_t s, int* iters_done, float* re) {
+              const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
     const int dim = nz <= 1 ? 2 : 3;
     if (dim == 2) nz = 1;
     *iters_done = 0;
@@ -83,7 +83,8 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
-    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
+    // returning the state after a stop needs the stop iteration's own pass
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
@@ -112,6 +113,14 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
         launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
         launch_zero(ar.at(oP[nbuf - 1][c]), n * sizeof(S), s);
     }
+    auto Pbuf = [&](int j, int c) -> S* { return (S*)ar.at(oP[((j % nbuf) + nbuf) % nbuf][c < dim ? c : 0]); };
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws && ws->warm)   // P_0 and P_{-1} from the caller's state
+            for (int c = 0; c < dim; ++c) {
+                state_in<S>(Pbuf(0, c), ws->d_state + (size_t)c * n, n, s);
+                state_in<S>(Pbuf(-1, c), ws->d_state + (size_t)(3 + c) * n, n, s);
+            }
+    }
     const bool use_tol = epsil != (T)0;
     T* Dbuf[2] = {(T*)ar.at(oD[0]), (T*)ar.at(oD[1])};
     if (use_tol) launch_copy(Dbuf[1], d_out, n * sizeof(T), s);   // D_{-1} = caller's Output
@@ -119,7 +128,6 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     double* partials0 = (double*)ar.at(oPart);
     double* partials1 = partials0 + 2 * (size_t)g.nblocks;
     const int ce = opts.check_every;
-    auto Pbuf = [&](int j, int c) -> S* { return (S*)ar.at(oP[((j % nbuf) + nbuf) % nbuf][c < dim ? c : 0]); };
     // D_k is needed when iteration k is the last, is checked, or precedes a check
     auto needs_d = [&](int k) {
         return k == iter - 1 || (use_tol && (k % ce == 0 || (k + 1) % ce == 0));
@@ -128,11 +136,16 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
 
     // t_k sequence in the working precision (FGP_TV_core.c:138,159): mult[k]
     // multiplies (P_k - P_{k-1}) in iteration k
-    std::vector<T> mult(iter + 1);
+    std::vector<T> mult(iter + 1), tks(iter + 1);
     {
         T tk = 1, m = 0;
+        if (ws && ws->warm) {   // resume the sequence where the caller's state left it
+            tk = (T)ws->tstate[0];
+            m = (T)ws->tstate[1];
+        }
         for (int k = 0; k <= iter; ++k) {
             mult[k] = m;
+            tks[k] = tk;
             const T tkp1 = ((T)1 + dsqrt_host(((T)1 + (T)4 * tk * tk))) * (T)0.5;
             m = (tk - (T)1) / tkp1;
             tk = tkp1;
@@ -200,16 +213,27 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     // the last D written: iteration `done` on a stop, else iter - 1
     const int last = (done < iter) ? done : iter - 1;
     CILREG_CK(hipMemcpyAsync(d_out, Dbuf[last & 1], n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws) {   // P and P_prev after the executed iterations, t and the next multiplier
+            const int executed = last + 1;
+            for (int c = 0; c < dim; ++c) {
+                state_out<S>(ws->d_state + (size_t)c * n, Pbuf(executed, c), n, s);
+                state_out<S>(ws->d_state + (size_t)(3 + c) * n, Pbuf(executed - 1, c), n, s);
+            }
+            ws->tstate[0] = (float)tks[executed];
+            ws->tstate[1] = (float)mult[executed];
+        }
+    }
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
 template int fgp_solve<float, float>(const float*, float*, float, int, float, int, int, int, int, int, const Opts&,
-                                     hipStream_t, int*, float*);
+                                     hipStream_t, int*, float*, const WarmState*);
 template int fgp_solve<float, bf16_t>(const float*, float*, float, int, float, int, int, int, int, int, const Opts&,
-                                      hipStream_t, int*, float*);
+                                      hipStream_t, int*, float*, const WarmState*);
 template int fgp_solve<float, _Float16>(const float*, float*, float, int, float, int, int, int, int, int,
-                                        const Opts&, hipStream_t, int*, float*);
+                                        const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int fgp_solve<double, double>(const double*, double*, double, int, double, int, int, int, int, int,
-                                       const Opts&, hipStream_t, int*, float*);
+                                       const Opts&, hipStream_t, int*, float*, const WarmState*);
 
 }  // namespace cilreg

@@ -69,7 +69,7 @@ static void launch_pd2(const Pd2Args<T, S>& a, bool iso, bool small, hipStream_t
 // un-extrapolated U in Ubrk0 (first iteration of a pass) or Ubrk1 (second).
 template <typename T, typename S>
 int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, int method, int nonneg, int nx, int ny,
-             int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re) {
+             int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
     const int dim = nz <= 1 ? 2 : 3;
     if (dim == 2) nz = 1;
     const size_t n = (size_t)nx * ny * nz;
@@ -83,7 +83,8 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
-    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
+    // returning the state after a stop needs the stop iteration's own pass
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
@@ -110,6 +111,12 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     if (ar.allocate() != 0) return -1;
     launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
     for (int c = 0; c < dim; ++c) launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws && ws->warm) {   // u_bar and P from the caller's state [u_bar, P1, P2, P3]
+            state_in<T>((T*)ar.at(oU[0]), ws->d_state, n, s);
+            for (int c = 0; c < dim; ++c) state_in<S>((S*)ar.at(oP[0][c]), ws->d_state + (size_t)(1 + c) * n, n, s);
+        }
+    }
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
     double* partials0 = (double*)ar.at(oPart);
     double* partials1 = partials0 + 2 * (size_t)g.nblocks;
@@ -195,14 +202,26 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     }
     if (!src) { set_error("PD: no pass covers iteration %d", done); return -1; }
     CILREG_CK(hipMemcpyAsync(d_out, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws) {   // the state set left by the pass holding the last executed iteration
+            const int last = done < iter ? done : iter - 1;
+            int set = -1;
+            for (size_t q = 0; q < pass_k.size(); ++q)
+                if (last >= pass_k[q] && last < pass_k[q] + pass_n[q]) set = pass_out[q];
+            if (set < 0) { set_error("PD: no pass covers iteration %d", last); return -1; }
+            state_out<T>(ws->d_state, (const T*)ar.at(oU[set]), n, s);
+            for (int c = 0; c < dim; ++c)
+                state_out<S>(ws->d_state + (size_t)(1 + c) * n, (const S*)ar.at(oP[set][c]), n, s);
+        }
+    }
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
 template int pd_solve<float, float>(const float*, float*, float, int, float, float, int, int, int, int, int,
-                                    const Opts&, hipStream_t, int*, float*);
+                                    const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int pd_solve<float, bf16_t>(const float*, float*, float, int, float, float, int, int, int, int, int,
-                                     const Opts&, hipStream_t, int*, float*);
+                                     const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int pd_solve<float, _Float16>(const float*, float*, float, int, float, float, int, int, int, int, int,
-                                       const Opts&, hipStream_t, int*, float*);
+                                       const Opts&, hipStream_t, int*, float*, const WarmState*);
 
 }  // namespace cilreg

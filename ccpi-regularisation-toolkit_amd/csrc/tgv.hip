@@ -64,7 +64,7 @@ static void launch_tgv2(const Tgv2Args<T, S>& a, bool small, hipStream_t s) {
 // rule fire there.
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
-              const Opts& opts, hipStream_t s, int* iters_done, float* re) {
+              const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
     const int dim = nz == 1 ? 2 : 3;
     const size_t n = (size_t)nx * ny * nz;
     *iters_done = 0;
@@ -77,7 +77,8 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     const int nq = dim == 3 ? 6 : 3;
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
-    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
+    // returning the state after a stop needs the stop iteration's own pass
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
@@ -110,6 +111,17 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
     }
     for (int c = 0; c < nq; ++c) launch_zero(ar.at(oQ[0][c]), n * sizeof(S), s);
+    // caller state channels: u_bar, V1..3, P1..3, Q (Q11, Q22, Q33, Q12, Q13, Q23 in 3D; Q11, Q22, Q12 in 2D)
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws && ws->warm) {
+            state_in<T>((T*)ar.at(oU[0]), ws->d_state, n, s);
+            for (int c = 0; c < dim; ++c) {
+                state_in<S>((S*)ar.at(oV[0][c]), ws->d_state + (size_t)(1 + c) * n, n, s);
+                state_in<S>((S*)ar.at(oP[0][c]), ws->d_state + (size_t)(4 + c) * n, n, s);
+            }
+            for (int c = 0; c < nq; ++c) state_in<S>((S*)ar.at(oQ[0][c]), ws->d_state + (size_t)(7 + c) * n, n, s);
+        }
+    }
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
     double* partials0 = (double*)ar.at(oPart);
     double* partials1 = partials0 + 2 * (size_t)g.nblocks;
@@ -201,17 +213,32 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     }
     if (!src) { set_error("TGV: no pass covers iteration %d", last); return -1; }
     CILREG_CK(hipMemcpyAsync(d_out, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if constexpr (std::is_same<T, float>::value) {
+        if (ws) {   // the state set left by the pass holding the last executed iteration
+            int set = -1;
+            for (size_t q = 0; q < pass_k.size(); ++q)
+                if (last >= pass_k[q] && last < pass_k[q] + pass_n[q]) set = pass_out[q];
+            if (set < 0) { set_error("TGV: no pass covers iteration %d", last); return -1; }
+            state_out<T>(ws->d_state, (const T*)ar.at(oU[set]), n, s);
+            for (int c = 0; c < dim; ++c) {
+                state_out<S>(ws->d_state + (size_t)(1 + c) * n, (const S*)ar.at(oV[set][c]), n, s);
+                state_out<S>(ws->d_state + (size_t)(4 + c) * n, (const S*)ar.at(oP[set][c]), n, s);
+            }
+            for (int c = 0; c < nq; ++c)
+                state_out<S>(ws->d_state + (size_t)(7 + c) * n, (const S*)ar.at(oQ[set][c]), n, s);
+        }
+    }
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
 
 template int tgv_solve<float, float>(const float*, float*, float, float, float, int, float, float, int, int, int,
-                                     const Opts&, hipStream_t, int*, float*);
+                                     const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int tgv_solve<float, bf16_t>(const float*, float*, float, float, float, int, float, float, int, int, int,
-                                      const Opts&, hipStream_t, int*, float*);
+                                      const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int tgv_solve<float, _Float16>(const float*, float*, float, float, float, int, float, float, int, int, int,
-                                        const Opts&, hipStream_t, int*, float*);
+                                        const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int tgv_solve<double, double>(const double*, double*, double, double, double, int, double, double, int, int,
-                                       int, const Opts&, hipStream_t, int*, float*);
+                                       int, const Opts&, hipStream_t, int*, float*, const WarmState*);
 
 }  // namespace cilreg

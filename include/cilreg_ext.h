@@ -120,6 +120,30 @@ CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, doubl
  * 2 reads + 1 write.  Bytes moved / kernel time, in GB/s. */
 CILREG_API int cilreg_stream_bench(size_t bytes, int reps, double *gbps);
 
+/* Warm start: the solver's iterate state in caller memory, so an outer loop
+ * (FISTA, ADMM) can resume the inner solver.  d_state holds FP32 channels of
+ * dimX*dimY*dimZ values each, whatever dual_precision is:
+ *   FGP: P1, P2, P3, P1_prev, P2_prev, P3_prev (2D: channels 0, 1, 3, 4);
+ *        t_state (host float[2]) = {t_k, multiplier of the next iteration};
+ *   PD:  u_bar, P1, P2, P3 (2D: 0, 1, 2);
+ *   TGV: u_bar, V1..V3, P1..P3, Q11, Q22, Q33, Q12, Q13, Q23
+ *        (2D: u_bar, V1, V2, P1, P2 in channels 0, 1, 2, 4, 5 and Q11, Q22, Q12 in 7, 8, 9).
+ * warm = 1 starts from the state (and t_state); warm = 0 starts cold like
+ * the plain solve.  On return the state is that after the executed
+ * iterations: all `iter`, or, with a tolerance, up to and including the
+ * iteration at which the stopping rule fired.  N iterations followed by a
+ * warm call of M equal one call of N + M bit for bit (fixed counts; the
+ * stopping rule's count restarts per call).  iter >= 1. */
+CILREG_API int cilreg_fgp_dev_ws(const float *d_input, float *d_output, float *infovector, float lambdaPar, int iter,
+                                 float epsil, int methodTV, int nonneg, int dimX, int dimY, int dimZ, float *d_state,
+                                 float *t_state, int warm, const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_pd_dev_ws(const float *d_input, float *d_output, float *infovector, float lambdaPar, int iter,
+                                float epsil, float lipschitz_const, int methodTV, int nonneg, int dimX, int dimY,
+                                int dimZ, float *d_state, int warm, const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_tgv_dev_ws(const float *d_input, float *d_output, float *infovector, float lambda, float alpha1,
+                                 float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ,
+                                 float *d_state, int warm, const cilreg_opts *opts, void *stream);
+
 /* The autotuner: times short fixed-count solves of a synthetic volume of the
  * given extent for each candidate tiling (tile_y 8/4, zchunk planned or
  * 8/16/32) and temporal depth (1/2 for FGP, PD, TGV in 3D) with base's dual
