@@ -1,7 +1,7 @@
 """ctypes wrappers of the TV-family regularisers.
 
 Same function names, argument order and defaults as the reference's
-src/Python/ccpi/filters/TV.py (GPU wrappers :477-733, CPU wrappers :8-250):
+src/Python/ccpi/filters/TV.py (GPU wrappers :477-771, CPU wrappers :8-292):
 numpy arrays in (Z, Y, X) or (Y, X) order, ``dims = shape[::-1]`` (+1 for 2D),
 ``out`` and ``infovector`` optional and filled in place, the output returned.
 
@@ -144,6 +144,29 @@ def TGV_GPU(inputData, lambdaPar, alpha1, alpha0, iterationsNumb, L2, epsil, gpu
     return out
 
 
+def _ref_image(x, inputRef):
+    r = _f32(inputRef)
+    if r.shape != x.shape:
+        raise ValueError(f"inputRef shape {r.shape} differs from the input's {x.shape}")
+    return r
+
+
+def dTV_FGP_GPU(inputData, inputRef, lambdaPar, iterationsNumb, epsil, eta, methodTV, nonneg, gpu_device, out=None,
+                infovector=None):
+    lib = require_gpu_lib()
+    lib.dTV_FGP_GPU_main.argtypes = [_fp, _fp, _fp, _fp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _i]
+    lib.dTV_FGP_GPU_main.restype = _i
+    x = _f32(inputData)
+    r = _ref_image(x, inputRef)
+    out = _out(x, out)
+    infovector = _info(infovector)
+    d = _dims(x)
+    rc = lib.dTV_FGP_GPU_main(_p(x), _p(r), _p(out), _p(infovector), lambdaPar, iterationsNumb, epsil, eta, methodTV,
+                              nonneg, _gpu_device(gpu_device), *d)
+    _check(rc, "dTV_FGP_GPU_main")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # CPU wrappers: the reference's own C-OMP library, when the user supplies it
 # ---------------------------------------------------------------------------
@@ -212,4 +235,18 @@ def TGV_CPU(inputData, lambdaPar, alpha1, alpha0, iterationsNumb, L2, epsil, out
     out = _out(x, out)
     infovector = _info(infovector)
     lib.TGV_main(_p(x), _p(out), _p(infovector), lambdaPar, alpha1, alpha0, iterationsNumb, L2, epsil, *_dims(x))
+    return out
+
+
+def dTV_FGP_CPU(inputData, inputRef, lambdaPar, iterationsNumb, epsil, eta, methodTV, nonneg, out=None,
+                infovector=None):
+    lib = _require_cpu_lib()
+    lib.dTV_FGP_CPU_main.argtypes = [_fp, _fp, _fp, _fp, _f, _i, _f, _f, _i, _i, _i, _i, _i]
+    lib.dTV_FGP_CPU_main.restype = _f
+    x = _f32(inputData)
+    r = _ref_image(x, inputRef)
+    out = _out(x, out)
+    infovector = _info(infovector)
+    lib.dTV_FGP_CPU_main(_p(x), _p(r), _p(out), _p(infovector), lambdaPar, iterationsNumb, epsil, eta, methodTV,
+                         nonneg, *_dims(x))
     return out

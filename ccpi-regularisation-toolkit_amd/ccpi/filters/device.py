@@ -94,6 +94,7 @@ _SIGS = {
     "cilreg_pd_dev": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_sb_dev": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_dtv_dev": [_vp, _vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev_f64": [_vp, _vp, _vp, _d, _d, _d, _i, _d, _d, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
@@ -172,6 +173,24 @@ def tgv(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
     import torch
     out = _prep(x, out, torch.float32)
     return _run("cilreg_tgv_dev", x, out, torch.float32, [lam, alpha1, alpha0, iters, L2, tol], make_opts(**kw))
+
+
+def dtv(x, ref, lam, iters, tol=0.0, eta=0.01, method=0, nonneg=0, out=None, **kw):
+    """FGP-dTV guided by the reference image ``ref`` (a tensor of x's shape, dtype and device)."""
+    import torch
+    out = _prep(x, out, torch.float32)
+    if not (ref.is_cuda and ref.dtype == torch.float32 and ref.is_contiguous() and ref.shape == x.shape
+            and ref.device == x.device):
+        raise ValueError("ref must be a contiguous float32 tensor of the input's shape and device")
+    info = np.zeros(2, dtype=np.float32)
+    f = _fn("cilreg_dtv_dev")
+    with torch.cuda.device(x.device):
+        rc = f(x.data_ptr(), ref.data_ptr(), out.data_ptr(), info.ctypes.data, lam, iters, tol, eta, method, nonneg,
+               *_dims(x), ctypes.byref(make_opts(**kw)),
+               ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"cilreg_dtv_dev failed: {last_error()}")
+    return out, info
 
 
 def fgp_f64(x, lam, iters, tol=0.0, method=0, nonneg=0, out=None, **kw):
@@ -276,4 +295,6 @@ def run(method, x, params, out=None, **kw):
         return sb(x, p["mu"], p["iters"], p["tol"], p["method"], out=out, **kw)
     if method == "tgv":
         return tgv(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], out=out, **kw)
+    if method == "dtv":   # p["ref"]: the reference image, a tensor of x's shape
+        return dtv(x, p["ref"], p["lambda"], p["iters"], p["tol"], p["eta"], p["method"], p["nonneg"], out=out, **kw)
     raise ValueError(method)

@@ -1,6 +1,6 @@
-"""User API: ``ROF_TV, FGP_TV, PD_TV, SB_TV, TGV`` with ``device=`` dispatch.
+"""User API: ``ROF_TV, FGP_TV, PD_TV, SB_TV, TGV, FGP_dTV`` with ``device=`` dispatch.
 
-Mirrors src/Python/ccpi/filters/regularisers.py:19-49 of the reference:
+Mirrors src/Python/ccpi/filters/regularisers.py:19-49 (FGP_dTV: :44) of the reference:
 ``device='cpu'`` calls the C-OMP function, ``'gpu'`` or an integer device
 index calls the MI355X function with ``gpu_device`` set, anything else raises
 KeyError.  Positional argument order per method:
@@ -10,6 +10,7 @@ KeyError.  Positional argument order per method:
 * ``PD_TV(x, lambda, iterations, tol, lipschitz_const, methodTV, nonneg)``
 * ``SB_TV(x, mu, iterations, tol, methodTV)``
 * ``TGV(x, lambda, alpha1, alpha0, iterations, L2, tol)``
+* ``FGP_dTV(x, x_ref, lambda, iterations, tol, eta, methodTV, nonneg)``
 
 each also taking ``out=`` and ``infovector=`` (``Output=`` for PD on the GPU,
 as in the reference, TV.py:567).
@@ -22,6 +23,7 @@ from .TV import TV_FGP_CPU, TV_FGP_GPU
 from .TV import PDTV_CPU, PDTV_GPU
 from .TV import SB_TV_CPU, SB_TV_GPU
 from .TV import TGV_CPU, TGV_GPU
+from .TV import dTV_FGP_CPU, dTV_FGP_GPU
 from .utils import cilregcuda
 
 
@@ -41,3 +43,4 @@ FGP_TV = create_wrapper(TV_FGP_CPU, TV_FGP_GPU)
 PD_TV = create_wrapper(PDTV_CPU, PDTV_GPU)
 SB_TV = create_wrapper(SB_TV_CPU, SB_TV_GPU)
 TGV = create_wrapper(TGV_CPU, TGV_GPU)
+FGP_dTV = create_wrapper(dTV_FGP_CPU, dTV_FGP_GPU)

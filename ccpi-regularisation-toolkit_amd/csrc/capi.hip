@@ -43,6 +43,35 @@ int host_call(int gpu_device, const float* in, float* out, float* info, size_t n
     return 0;
 }
 
+// host_call with a second input uploaded beside the first (FGP-dTV's
+// reference image); solve receives both device pointers.
+template <class Solve>
+int host_call_ref(int gpu_device, const float* in, const float* ref, float* out, float* info, size_t n,
+                  bool upload_out, bool skip_download, Solve&& solve) {
+    if (select_device(gpu_device) != 0) return -1;
+    hipStream_t s = device_stream();
+    if (!s) { set_error("cannot create a stream on device %d", gpu_device); return -1; }
+    Arena ar;
+    const size_t oin = ar.reserve(n * sizeof(float));
+    const size_t oref = ar.reserve(n * sizeof(float));
+    const size_t oout = ar.reserve(n * sizeof(float));
+    if (ar.allocate() != 0) return -1;
+    float* d_in = (float*)ar.at(oin);
+    float* d_ref = (float*)ar.at(oref);
+    float* d_out = (float*)ar.at(oout);
+    CILREG_CK(hipMemcpyAsync(d_in, in, n * sizeof(float), hipMemcpyHostToDevice, s));
+    CILREG_CK(hipMemcpyAsync(d_ref, ref, n * sizeof(float), hipMemcpyHostToDevice, s));
+    if (upload_out) CILREG_CK(hipMemcpyAsync(d_out, out, n * sizeof(float), hipMemcpyHostToDevice, s));
+    int done = 0;
+    float re = 0.f;
+    if (solve(d_in, d_ref, d_out, s, &done, &re) != 0) return -1;
+    if (!skip_download) CILREG_CK(hipMemcpyAsync(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    CILREG_CK(hipStreamSynchronize(s));
+    info[0] = (float)done;
+    info[1] = re;
+    return 0;
+}
+
 bool dims_ok(int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz < 0) {
         set_error("invalid dimensions (%d, %d, %d)", nx, ny, nz);
@@ -154,6 +183,20 @@ int TGV_GPU_main(float* U0, float* U, float* infovector, float lambda, float alp
                      });
 }
 
+int dTV_FGP_GPU_main(float* Input, float* InputRef, float* Output, float* infovector, float lambdaPar, int iter,
+                     float epsil, float eta, int methodTV, int nonneg, int gpu_device, int dimX, int dimY, int dimZ) {
+    if (!dims_ok(dimX, dimY, dimZ)) return -1;
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    const size_t n = (size_t)dimX * dimY * nz;
+    Opts o;
+    // like FGP, Output is left untouched when no iteration runs (FGP_dTV_core.c:76)
+    return host_call_ref(gpu_device, Input, InputRef, Output, infovector, n, epsil != 0.0f, iter <= 0,
+                         [&](const float* di, const float* dr, float* dout, hipStream_t s, int* done, float* re) {
+                             return dtv_solve<float>(di, dr, dout, lambdaPar, iter, epsil, eta, methodTV, nonneg, dimX,
+                                                     dimY, nz, o, s, done, re);
+                         });
+}
+
 // ---------------------------------------------------------------------------
 // device-resident extensions
 // ---------------------------------------------------------------------------
@@ -236,6 +279,17 @@ int cilreg_tgv_dev(const float* d_input, float* d_output, float* infovector, flo
     else
         rc = tgv_solve<float, float>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o,
                                      s, &done, &re);
+    DEV_EPILOGUE()
+}
+
+int cilreg_dtv_dev(const float* d_input, const float* d_ref, float* d_output, float* infovector, float lambdaPar,
+                   int iter, float epsil, float eta, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
+                   const cilreg_opts* opts, void* stream) {
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    if (o.precision != CILREG_FP32) { set_error("FGP-dTV keeps its fields in FP32 (dual_precision must be 0)"); return -1; }
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    rc = dtv_solve<float>(d_input, d_ref, d_output, lambdaPar, iter, epsil, eta, methodTV, nonneg, dimX, dimY, nz, o, s,
+                          &done, &re);
     DEV_EPILOGUE()
 }
 

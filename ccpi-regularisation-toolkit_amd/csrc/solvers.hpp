@@ -29,5 +29,8 @@ int sb_solve(const T* d_f, T* d_out, T mu, int iter, T epsil, int method, int nx
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws = nullptr);
+template <typename T>
+int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsil, T eta, int method, int nonneg,
+              int nx, int ny, int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);
 
 }  // namespace cilreg

@@ -105,6 +105,11 @@ CILREG_API int cilreg_sb_dev(const float *d_input, float *d_output, float *infov
 CILREG_API int cilreg_tgv_dev(const float *d_input, float *d_output, float *infovector, float lambda, float alpha1,
                    float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ,
                    const cilreg_opts *opts, void *stream);
+/* FGP-dTV on device buffers (d_ref: the reference image, same dimensions);
+ * FP32 storage only (dual_precision must be 0). */
+CILREG_API int cilreg_dtv_dev(const float *d_input, const float *d_ref, float *d_output, float *infovector,
+                   float lambdaPar, int iter, float epsil, float eta, int methodTV, int nonneg, int dimX, int dimY,
+                   int dimZ, const cilreg_opts *opts, void *stream);
 
 /* FP64 solves (compute and storage in double; dual_precision is ignored). */
 CILREG_API int cilreg_fgp_dev_f64(const double *d_input, double *d_output, double *infovector, double lambdaPar, int iter,

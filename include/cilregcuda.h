@@ -61,6 +61,16 @@ CILREG_API int SB_TV_GPU_main(float *Input, float *Output, float *infovector, fl
 CILREG_API int TGV_GPU_main(float *U0, float *U, float *infovector, float lambda, float alpha1, float alpha0,
                  int iterationsNumb, float L2, float epsil, int gpu_device, int dimX, int dimY, int dimZ);
 
+/* FGP-dTV: FGP-TV guided by the normalised gradient of a reference image
+ * InputRef of the same dimensions (eta smooths that gradient's magnitude).
+ * Replaces src/Core/regularisers_GPU/dTV_FGP_GPU_core.h:7.  2D when
+ * dimZ <= 1.  With epsil != 0 the first residual compares against the
+ * caller's Output contents, as the C-OMP solver does; Output is left
+ * untouched when iter <= 0. */
+CILREG_API int dTV_FGP_GPU_main(float *Input, float *InputRef, float *Output, float *infovector, float lambdaPar,
+                     int iter, float epsil, float eta, int methodTV, int nonneg, int gpu_device, int dimX, int dimY,
+                     int dimZ);
+
 #ifdef __cplusplus
 }
 #endif

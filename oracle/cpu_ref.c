@@ -338,6 +338,145 @@ REAL FN(cr_fgp)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPa
 }
 
 /* ------------------------------------------------------------------------ */
+/* FGP-dTV: FGP whose dual field and objective gradient are projected onto  */
+/* the orthogonal complement of the reference image's eta-normalised        */
+/* gradient (directional TV).  FGP_dTV_core.c:46-380                         */
+/* ------------------------------------------------------------------------ */
+
+/* B = grad(ref) / sqrt(|grad(ref)|^2 + eta^2), forward differences that take
+ * the value past the last index as 0 (so -ref there, not 0); the sum is in the
+ * working precision, its square root in double (FGP_dTV_core.c:188-208,
+ * :279-306). */
+static void dtv_refgrad(const REAL *ref, REAL *B1, REAL *B2, REAL *B3, REAL eta, idx_t nx, idx_t ny, idx_t nz,
+                        int ndim)
+{
+    REAL eta_sq = eta * eta;
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                idx_t c = sxy * k + nx * j + i;
+                REAL v1 = (i == nx - 1) ? 0 : ref[c + 1];
+                REAL v2 = (j == ny - 1) ? 0 : ref[c + nx];
+                REAL g1 = v1 - ref[c], g2 = v2 - ref[c], g3 = 0;
+                REAL m = g1 * g1 + g2 * g2;
+                if (ndim == 3) {
+                    REAL v3 = (k == nz - 1) ? 0 : ref[c + sxy];
+                    g3 = v3 - ref[c];
+                    m = m + g3 * g3;
+                }
+                m = (REAL)sqrt((double)(m + eta_sq));
+                B1[c] = g1 / m;
+                B2[c] = g2 / m;
+                if (ndim == 3) B3[c] = g3 / m;
+            }
+    }
+}
+
+/* v <- v - <v, B> B pointwise (ProjectVect, FGP_dTV_core.c:210-223, :308-323). */
+static void dtv_project_dual(REAL *R1, REAL *R2, REAL *R3, const REAL *B1, const REAL *B2, const REAL *B3, int ndim,
+                             idx_t n)
+{
+    idx_t t;
+#pragma omp parallel for
+    for (t = 0; t < n; t++) {
+        REAL ip = R1[t] * B1[t] + R2[t] * B2[t];
+        if (ndim == 3) ip = ip + R3[t] * B3[t];
+        R1[t] = R1[t] - ip * B1[t];
+        R2[t] = R2[t] - ip * B2[t];
+        if (ndim == 3) R3[t] = R3[t] - ip * B3[t];
+    }
+}
+
+/* P = R + step * (g - <g, B> B) with g_d = D(x) - D(x + e_d), 0 at the last
+ * index (Grad_dfunc, FGP_dTV_core.c:240-262, :342-367). */
+static void dtv_gradient(REAL *P1, REAL *P2, REAL *P3, const REAL *D, const REAL *R1, const REAL *R2, const REAL *R3,
+                         const REAL *B1, const REAL *B2, const REAL *B3, REAL step, idx_t nx, idx_t ny, idx_t nz,
+                         int ndim)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                idx_t c = sxy * k + nx * j + i;
+                REAL g1 = (i == nx - 1) ? 0 : D[c] - D[c + 1];
+                REAL g2 = (j == ny - 1) ? 0 : D[c] - D[c + nx];
+                REAL g3 = 0, ip;
+                if (ndim == 3) g3 = (k == nz - 1) ? 0 : D[c] - D[c + sxy];
+                ip = g1 * B1[c] + g2 * B2[c];
+                if (ndim == 3) ip = ip + g3 * B3[c];
+                g1 = g1 - ip * B1[c];
+                g2 = g2 - ip * B2[c];
+                P1[c] = R1[c] + step * g1;
+                P2[c] = R2[c] + step * g2;
+                if (ndim == 3) {
+                    g3 = g3 - ip * B3[c];
+                    P3[c] = R3[c] + step * g3;
+                }
+            }
+    }
+}
+
+REAL FN(cr_dtv)(const REAL *Input, const REAL *InputRef, REAL *Output, REAL *infovector, REAL lambdaPar,
+                int iterationsNumb, REAL epsil, REAL eta, int methodTV, int nonneg, int dimX, int dimY, int dimZ)
+{
+    int ndim = dimZ <= 1 ? 2 : 3;
+    idx_t nz = ndim == 2 ? 1 : dimZ;
+    idx_t n = (idx_t)dimX * dimY * nz, t;
+    idx_t alloc = n ? n : 1;
+    REAL *prev = NULL;
+    REAL *P[3], *Pold[3], *R[3], *B[3];
+    REAL tk = 1, tkp1 = 1;
+    /* 1/(8 lambda) in 2D, 1/(26 lambda) in 3D (FGP_dTV_core.c:244, :346) */
+    REAL step = (REAL)1 / ((ndim == 3 ? (REAL)26 : (REAL)8) * lambdaPar);
+    stop_state st = {0, 0};
+    int it, d;
+
+    for (d = 0; d < 3; d++) {
+        P[d] = calloc(alloc, sizeof(REAL));
+        Pold[d] = calloc(alloc, sizeof(REAL));
+        R[d] = calloc(alloc, sizeof(REAL));
+        B[d] = calloc(alloc, sizeof(REAL));
+    }
+    if (epsil != 0) prev = calloc(alloc, sizeof(REAL));
+    dtv_refgrad(InputRef, B[0], B[1], B[2], eta, dimX, dimY, nz, ndim);
+
+    /* iteration order of FGP_dTV_core.c:76-113 (2D) and :129-169 (3D) */
+    for (it = 0; it < iterationsNumb; it++) {
+        int check = (epsil != 0) && (it % 5 == 0);
+        REAL m;
+        if (check) copy_field(Output, prev, n);
+        dtv_project_dual(R[0], R[1], R[2], B[0], B[1], B[2], ndim, n);
+        fgp_objective(Input, Output, R[0], R[1], R[2], lambdaPar, dimX, dimY, nz, ndim);
+        if (nonneg == 1)
+            for (t = 0; t < n; t++) if (Output[t] < 0) Output[t] = 0;
+        dtv_gradient(P[0], P[1], P[2], Output, R[0], R[1], R[2], B[0], B[1], B[2], step, dimX, dimY, nz, ndim);
+        project_unit(P[0], P[1], P[2], ndim, methodTV != 0, n);
+        /* t_{k+1} with a double square root and a double sum, rounded once
+         * (FGP_dTV_core.c:95, :149) */
+        tkp1 = (REAL)(((double)1 + sqrt((double)((REAL)1 + (REAL)4 * tk * tk))) * 0.5);
+        m = (tk - (REAL)1) / tkp1;
+#pragma omp parallel for private(d)
+        for (t = 0; t < n; t++)
+            for (d = 0; d < ndim; d++) R[d][t] = P[d][t] + m * (P[d][t] - Pold[d][t]);
+        for (d = 0; d < ndim; d++) copy_field(P[d], Pold[d], n);
+        tk = tkp1;
+        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+    }
+    for (d = 0; d < 3; d++) { free(P[d]); free(Pold[d]); free(R[d]); free(B[d]); }
+    free(prev);
+    infovector[0] = (REAL)it;
+    infovector[1] = st.re;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* PD-TV: Chambolle-Pock primal-dual.  PD_TV_core.c:40-252                   */
 /* ------------------------------------------------------------------------ */
 

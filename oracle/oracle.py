@@ -64,19 +64,22 @@ class _Lib:
             self.f_pd = getattr(self.lib, f"cr_pd_{prec}")
             self.f_sb = getattr(self.lib, f"cr_sb_{prec}")
             self.f_tgv = getattr(self.lib, f"cr_tgv_{prec}")
+            self.f_dtv = getattr(self.lib, f"cr_dtv_{prec}")
         else:  # the reference's own symbol names (src/Core/regularisers_CPU/*.h)
             self.f_rof = self.lib.TV_ROF_CPU_main
             self.f_fgp = self.lib.TV_FGP_CPU_main
             self.f_pd = self.lib.PDTV_CPU_main
             self.f_sb = self.lib.SB_TV_CPU_main
             self.f_tgv = self.lib.TGV_main
+            self.f_dtv = self.lib.dTV_FGP_CPU_main
         i = ctypes.c_int
         self.f_rof.argtypes = [ptr, ptr, ptr, ptr, i, i, real, real, i, i, i]
         self.f_fgp.argtypes = [ptr, ptr, ptr, real, i, real, i, i, i, i, i]
         self.f_pd.argtypes = [ptr, ptr, ptr, real, i, real, real, i, i, i, i, i]
         self.f_sb.argtypes = [ptr, ptr, ptr, real, i, real, i, i, i, i]
         self.f_tgv.argtypes = [ptr, ptr, ptr, real, real, real, i, real, real, i, i, i]
-        for f in (self.f_rof, self.f_fgp, self.f_pd, self.f_sb, self.f_tgv):
+        self.f_dtv.argtypes = [ptr, ptr, ptr, ptr, real, i, real, real, i, i, i, i, i]
+        for f in (self.f_rof, self.f_fgp, self.f_pd, self.f_sb, self.f_tgv, self.f_dtv):
             f.restype = real
 
     # -- helpers --------------------------------------------------------
@@ -127,6 +130,15 @@ class _Lib:
         self.f_tgv(self._p(x), self._p(out), self._p(info), lam, a1, a0, iters, L2, tol, *d)
         return out, info
 
+    def dtv(self, x, ref, lam, iters, tol, eta, method, nonneg, out=None):
+        x, out, info = self._prep(x, out)
+        r = np.ascontiguousarray(ref, dtype=self.dtype)
+        if r.shape != x.shape:
+            raise ValueError("reference image shape differs from the input's")
+        d = _dims(x)
+        self.f_dtv(self._p(x), self._p(r), self._p(out), self._p(info), lam, iters, tol, eta, method, nonneg, *d)
+        return out, info
+
     def run(self, method, x, params, out=None):
         """Dispatch by method name with a parameter dict (keys as in the
         reference Python API)."""
@@ -141,6 +153,9 @@ class _Lib:
             return self.sb(x, p["mu"], p["iters"], p["tol"], p["method"], out=out)
         if method == "tgv":
             return self.tgv(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], out=out)
+        if method == "dtv":   # p["ref"]: the reference image, an array of x's shape
+            return self.dtv(x, p["ref"], p["lambda"], p["iters"], p["tol"], p["eta"], p["method"], p["nonneg"],
+                            out=out)
         raise ValueError(method)
 
 

@@ -52,6 +52,14 @@ class Golden:
     def input(self, case):
         return self.arrays["in_" + case["input"]]
 
+    def params(self, case):
+        """The case's parameters; FGP-dTV cases also get their reference image as ``ref``."""
+        p = dict(case["params"])
+        k = case.get("ref_input")
+        if k is not None:
+            p["ref"] = self.arrays["in_" + k]
+        return p
+
     def out_init(self, case):
         k = case.get("out_init")
         return None if k is None else self.arrays["in_" + k].copy()

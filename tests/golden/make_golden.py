@@ -47,6 +47,14 @@ def inputs():
     rng = np.random.Generator(np.random.PCG64(17))
     noise = phantoms.gaussian_noise(rng, crop.size, 1.0).reshape(crop.shape)
     ins["lena96"] = (crop + 0.05 * crop * noise).astype(np.float32)
+    # reference images for FGP-dTV: the clean phantoms / crop with 1% noise, as
+    # the reference's dTV test pairs u0 with u_ref (test_2d_cpu_vs_gpu.py:28-44)
+    ins["p2d_ref"] = phantoms.noisy(phantoms.shapes_2d((37, 53), 8, seed=11), 0.01, seed=21)
+    ins["p3d_ref"] = phantoms.noisy(phantoms.ellipsoids_3d((11, 13, 17), 8, seed=13), 0.01, seed=22)
+    ins["p3d_big_ref"] = phantoms.noisy(phantoms.ellipsoids_3d((24, 40, 56), 12, seed=15), 0.01, seed=23)
+    rng2 = np.random.Generator(np.random.PCG64(24))
+    noise2 = phantoms.gaussian_noise(rng2, crop.size, 1.0).reshape(crop.shape)
+    ins["lena96_ref"] = (crop + 0.01 * crop * noise2).astype(np.float32)
     # a non-zero initial Output for FGP's first residual (FGP_TV_core.c:66,123)
     ins["p2d_init"] = (0.5 * ins["p2d"]).astype(np.float32)
     ins["p3d_init"] = (0.5 * ins["p3d"]).astype(np.float32)
@@ -85,6 +93,21 @@ def cases():
     cs.append(("pd", lena, {"lambda": 0.02, "iters": 1500, "tol": 0.0, "L": 8.0, "method": 0, "nonneg": 0}))
     cs.append(("sb", lena, {"mu": 0.02, "iters": 250, "tol": 0.0, "method": 0}))
     cs.append(("tgv", lena, {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 1000, "L2": 12.0, "tol": 0.0}))
+    # FGP-dTV (appended, so the ids above stay fixed)
+    for inp in ("p2d", "p3d"):
+        for method in (0, 1):
+            for nonneg in (0, 1):
+                cs.append(("dtv", inp, {"lambda": 0.02, "iters": 40, "tol": 0.0, "eta": 0.01, "method": method,
+                                        "nonneg": nonneg, "ref_input": inp + "_ref"}))
+                cs.append(("dtv", inp, {"lambda": 0.05, "iters": 300, "tol": 1e-3, "eta": 0.2, "method": method,
+                                        "nonneg": nonneg, "ref_input": inp + "_ref"}))
+        cs.append(("dtv", inp, {"lambda": 0.03, "iters": 100, "tol": 1e-3, "eta": 0.05, "method": 0, "nonneg": 0,
+                                "ref_input": inp + "_ref", "out_init": inp + "_init"}))
+    cs.append(("dtv", big, {"lambda": 0.02, "iters": 40, "tol": 1e-6, "eta": 0.01, "method": 0, "nonneg": 1,
+                            "ref_input": big + "_ref"}))
+    # the reference's dTV test parameters (test_2d_cpu_vs_gpu.py:556-566), fewer iterations
+    cs.append(("dtv", lena, {"lambda": 0.02, "iters": 300, "tol": 0.0, "eta": 0.2, "method": 0, "nonneg": 0,
+                             "ref_input": lena + "_ref"}))
     return cs
 
 
@@ -98,11 +121,16 @@ def main():
     for idx, (method, inp, params) in enumerate(cases()):
         p = dict(params)
         init_key = p.pop("out_init", None)
+        ref_key = p.pop("ref_input", None)
         out = ins[init_key].copy() if init_key else None
-        y, info = ref.run(method, ins[inp], p, out=out)
+        run_p = dict(p, ref=ins[ref_key]) if ref_key else p
+        y, info = ref.run(method, ins[inp], run_p, out=out)
         arrays[f"c{idx}_out"] = y
         arrays[f"c{idx}_info"] = info
-        table.append({"id": idx, "method": method, "input": inp, "params": p, "out_init": init_key})
+        row = {"id": idx, "method": method, "input": inp, "params": p, "out_init": init_key}
+        if ref_key:
+            row["ref_input"] = ref_key
+        table.append(row)
         print(idx, method, inp, p, "info", info.tolist())
     np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **arrays)
     with open(os.path.join(HERE, "ref_cases.json"), "w") as fh:

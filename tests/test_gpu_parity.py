@@ -26,7 +26,7 @@ def _gpu_run(case):
     from ccpi.filters import regularisers as R
     g = golden()
     x = g.input(case)
-    p = case["params"]
+    p = g.params(case)
     m = case["method"]
     info = np.zeros(2, dtype=np.float32)
     out = g.out_init(case)
@@ -40,10 +40,21 @@ def _gpu_run(case):
                     infovector=info)
     elif m == "sb":
         y = R.SB_TV(x, p["mu"], p["iters"], p["tol"], p["method"], device="gpu", infovector=info)
+    elif m == "dtv":
+        y = R.FGP_dTV(x, p["ref"], p["lambda"], p["iters"], p["tol"], p["eta"], p["method"], p["nonneg"], device="gpu",
+                      infovector=info, out=out)
     else:
         y = R.TGV(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], device="gpu",
                   infovector=info)
     return y, info
+
+
+def _device_params(g, case):
+    p = g.params(case)
+    if "ref" in p:
+        import torch
+        p["ref"] = torch.from_numpy(p["ref"]).cuda()
+    return p
 
 
 def _assert_parity(case, y, info):
@@ -80,7 +91,7 @@ def test_device_api_matches_reference(case):
     init = g.out_init(case)
     if init is not None:
         out = torch.from_numpy(init).cuda()
-    y, info = D.run(case["method"], x, case["params"], out=out)
+    y, info = D.run(case["method"], x, _device_params(g, case), out=out)
     _assert_parity(case, y.cpu().numpy(), info)
 
 
@@ -99,6 +110,10 @@ def test_tiling_does_not_change_results(tile_y, zchunk):
         a, _ = D.run(m, x, p)
         b, _ = D.run(m, x, p, tile_y=tile_y, zchunk=zchunk)
         assert torch.equal(a, b), (m, tile_y, zchunk)
+    r = torch.from_numpy(g.arrays["in_p3d_big_ref"]).cuda()
+    a, _ = D.dtv(x, r, 0.02, 12, 0.0, 0.01, 0, 1)
+    b, _ = D.dtv(x, r, 0.02, 12, 0.0, 0.01, 0, 1, tile_y=tile_y, zchunk=zchunk)
+    assert torch.equal(a, b), ("dtv", tile_y, zchunk)
 
 
 @pytest.mark.gpu
@@ -110,7 +125,7 @@ def test_wide_index_kernels_match(case):
     from ccpi.filters import device as D
     g = golden()
     x = torch.from_numpy(g.input(case)).cuda()
-    y, info = D.run(case["method"], x, case["params"], wide_index=True)
+    y, info = D.run(case["method"], x, _device_params(g, case), wide_index=True)
     _assert_parity(case, y.cpu().numpy(), info)
 
 
@@ -124,9 +139,11 @@ def test_zero_iterations():
         y = fn(x, *args, device="gpu", infovector=info)
         assert bits_equal(y, x)
         assert info[0] == 0
-    # FGP leaves Output untouched
+    # FGP and FGP-dTV leave Output untouched
     out = np.full_like(x, 7.0)
     y = R.FGP_TV(x, 0.02, 0, 0.0, 0, 0, device="gpu", out=out)
+    assert np.all(y == 7.0)
+    y = R.FGP_dTV(x, x, 0.02, 0, 0.0, 0.01, 0, 0, device="gpu", out=out)
     assert np.all(y == 7.0)
 
 
@@ -136,3 +153,33 @@ def test_invalid_device_raises():
     x = golden().arrays["in_p2d"]
     with pytest.raises(RuntimeError):
         R.FGP_TV(x, 0.02, 5, 0.0, 0, 0, device=99)
+
+
+DTV_SHAPES = [(1, 1), (1, 70), (65, 1), (5, 67, 3), (2, 3, 130), (9, 1, 66), (1, 9, 66), (19, 13, 131)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", DTV_SHAPES, ids=[str(s) for s in DTV_SHAPES])
+@pytest.mark.parametrize("method,nonneg", [(0, 1), (1, 0)])
+def test_dtv_ragged_shapes_match_oracle(shape, method, nonneg):
+    """Ragged and degenerate extents (partial tiles, singleton axes, one
+    plane per chunk) against the oracle, bit for bit."""
+    import zlib
+    import oracle
+    from ccpi.filters import regularisers as R
+    rng = np.random.default_rng(zlib.crc32(repr((shape, method)).encode()))
+    x = (rng.random(shape) * 2 - 0.5).astype(np.float32)
+    r = rng.random(shape).astype(np.float32)
+    y_ref, info_ref = oracle.cpu_ref().dtv(x, r, 0.04, 30, 0.0, 0.05, method, nonneg)
+    info = np.zeros(2, dtype=np.float32)
+    y = R.FGP_dTV(x, r, 0.04, 30, 0.0, 0.05, method, nonneg, device="gpu", infovector=info)
+    assert info[0] == info_ref[0]
+    assert bits_equal(y, y_ref), f"max |diff| {np.abs(y - y_ref).max():.3e}"
+
+
+@pytest.mark.gpu
+def test_dtv_rejects_mismatched_reference():
+    from ccpi.filters import regularisers as R
+    x = golden().arrays["in_p2d"]
+    with pytest.raises(ValueError):
+        R.FGP_dTV(x, x[:-1], 0.02, 5, 0.0, 0.01, 0, 0, device="gpu")

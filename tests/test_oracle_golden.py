@@ -24,7 +24,7 @@ def test_oracle_matches_reference_fixture(case):
     g = golden()
     x = g.input(case)
     out0 = g.out_init(case)
-    y, info = oracle.cpu_ref().run(case["method"], x, case["params"], out=out0)
+    y, info = oracle.cpu_ref().run(case["method"], x, g.params(case), out=out0)
     y_ref, info_ref = g.expected(case)
     assert bits_equal(y, y_ref), f"max |diff| = {np.abs(y - y_ref).max()}"
     assert bits_equal(info, info_ref), f"infovector {info} vs reference {info_ref}"
@@ -55,6 +55,26 @@ def test_oracle_matches_reference_library(shape, method, params):
     x = (rng.random(shape) * 2 - 0.5).astype(np.float32)   # negative values exercise nonneg
     a, ia = oracle.cpu_ref().run(method, x, params)
     b, ib = ref.run(method, x, params)
+    assert bits_equal(a, b)
+    assert bits_equal(ia, ib)
+
+
+DTV_EXTRA = [
+    {"lambda": 0.04, "iters": 25, "tol": 0.0, "eta": 0.02, "method": 0, "nonneg": 1},
+    {"lambda": 0.01, "iters": 120, "tol": 5e-3, "eta": 0.3, "method": 1, "nonneg": 0},
+]
+
+
+@pytest.mark.parametrize("shape", [(29, 31), (9, 10, 23), (3, 5, 7), (1, 9), (4, 1, 6)])
+@pytest.mark.parametrize("params", DTV_EXTRA, ids=["dtv0", "dtv1"])
+def test_oracle_dtv_matches_reference_library(shape, params):
+    ref = _ref_or_skip()
+    rng = np.random.default_rng(zlib.crc32(repr((shape, "dtv", params)).encode()))
+    x = (rng.random(shape) * 2 - 0.5).astype(np.float32)
+    r = rng.random(shape).astype(np.float32)
+    p = dict(params, ref=r)
+    a, ia = oracle.cpu_ref().run("dtv", x, p)
+    b, ib = ref.run("dtv", x, p)
     assert bits_equal(a, b)
     assert bits_equal(ia, ib)
 

@@ -13,6 +13,12 @@ reference fixtures.
   the remaining terms are summed in the same order.  A transposed operand or a
   wrong component in the symmetric gradient breaks this.
 * Zero iterations return the input (the FGP output buffer is left alone).
+* FGP-dTV: scale equivariance in (f, lambda) as FGP; the reference image's
+  normalised gradient is unchanged by scaling (ref, eta) together by 2, so the
+  output is too (a misplaced eta or a wrong square-root argument breaks it);
+  with a zero reference image the gradient field B is zero, no projection
+  happens, and 2D dTV equals 2D FGP (same 1/(8 lambda) step) up to the
+  rounding of t_k, which dTV takes through a double square root.
 * The FP64 build agrees with the FP32 build to FP32 accuracy.
 """
 import numpy as np
@@ -118,3 +124,34 @@ def test_tv_energy_matches_direct_sum():
     gz = np.diff(ud, axis=0, append=ud[-1:, :, :])
     direct = np.sum(2 * 0.1 * np.sqrt(gx ** 2 + gy ** 2 + gz ** 2)) + np.sum((ud - u0) ** 2)
     assert abs(e - direct) <= 1e-9 * abs(direct)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+@pytest.mark.parametrize("method,nonneg", [(0, 0), (1, 1)])
+def test_dtv_scale_equivariance(f, method, nonneg):
+    lib = oracle.cpu_ref()
+    ref = np.flip(f).copy()
+    a, ia = lib.dtv(f, ref, 0.03, 40, 0.0, 0.05, method, nonneg)
+    b, ib = lib.dtv(2 * f, ref, 0.06, 40, 0.0, 0.05, method, nonneg)
+    assert bits_equal(b, 2 * a)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+def test_dtv_reference_scale_invariance(f):
+    lib = oracle.cpu_ref()
+    ref = (np.roll(f, 3) * 1.7).astype(np.float32)
+    a, _ = lib.dtv(f, ref, 0.03, 30, 0.0, 0.1, 0, 0)
+    b, _ = lib.dtv(f, 2 * ref, 0.03, 30, 0.0, 0.2, 0, 0)
+    assert bits_equal(a, b)
+    c, _ = lib.dtv(f, ref, 0.03, 30, 0.0, 0.2, 0, 0)   # eta alone does matter
+    assert not bits_equal(a, c)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_dtv_zero_reference_is_fgp_2d(method):
+    lib = oracle.cpu_ref()
+    a, _ = lib.dtv(F2, np.zeros_like(F2), 0.03, 60, 0.0, 0.1, method, 0)
+    b, _ = lib.fgp(F2, 0.03, 60, 0.0, method, 0)
+    err = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
+    assert err < 1e-6, err
+    assert not np.allclose(a, F2, atol=1e-3)   # and it did smooth
