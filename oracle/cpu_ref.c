@@ -135,21 +135,27 @@ static REAL minmod_sq(REAL a, REAL b)
  * the start maps to +1.  Only the 3D D1 takes a double sqrt of the double
  * sum (ROF_TV_core.c:136); every other variant rounds the double sum to
  * float and calls sqrtf (:160,199,223,262). */
-static void rof_dir(const REAL *u, REAL *D, idx_t nx, idx_t ny, idx_t nz, int fwd, int ndim)
+/* Mirrored neighbours: +1, or -1 at the last index; -1, or +1 at index 0.
+ * An axis of extent 1 has no mirror image (the reference reads index -1
+ * there); it maps to the point itself. */
+static idx_t mir_p(idx_t i, idx_t n) { return i + 1 < n ? i + 1 : (i - 1 >= 0 ? i - 1 : 0); }
+static idx_t mir_m(idx_t i, idx_t n) { return i - 1 >= 0 ? i - 1 : (i + 1 < n ? i + 1 : 0); }
+
+static void rof_dir(const REAL *u, REAL *D, idx_t nx, idx_t ny, idx_t nz, int fwd, int ndim, double eps)
 {
     idx_t k;
     idx_t sxy = nx * ny;
 #pragma omp parallel for
     for (k = 0; k < nz; k++) {
         idx_t j, i;
-        idx_t kp = (k + 1 >= nz) ? k - 1 : k + 1;
-        idx_t km = (k - 1 < 0) ? k + 1 : k - 1;
+        idx_t kp = mir_p(k, nz);
+        idx_t km = mir_m(k, nz);
         for (j = 0; j < ny; j++) {
-            idx_t jp = (j + 1 >= ny) ? j - 1 : j + 1;
-            idx_t jm = (j - 1 < 0) ? j + 1 : j - 1;
+            idx_t jp = mir_p(j, ny);
+            idx_t jm = mir_m(j, ny);
             for (i = 0; i < nx; i++) {
-                idx_t ip = (i + 1 >= nx) ? i - 1 : i + 1;
-                idx_t im = (i - 1 < 0) ? i + 1 : i - 1;
+                idx_t ip = mir_p(i, nx);
+                idx_t im = mir_m(i, nx);
                 idx_t c = sxy * k + nx * j + i;
                 REAL uc = u[c];
                 /* forward / backward one-sided differences per axis */
@@ -172,7 +178,7 @@ static void rof_dir(const REAL *u, REAL *D, idx_t nx, idx_t ny, idx_t nz, int fw
                     REAL fk = u[sxy * kp + nx * j + i] - uc, bk = uc - u[sxy * km + nx * j + i];
                     s = s + minmod_sq(fk, bk);
                 }
-                sd = (double)s + ROF_EPS;
+                sd = (double)s + eps;
                 if (ndim == 3 && fwd == 0)
                     D[c] = num / (REAL)sqrt(sd);
                 else
@@ -192,11 +198,11 @@ static void rof_update(const REAL *D1, const REAL *D2, const REAL *D3, REAL *u, 
 #pragma omp parallel for
     for (k = 0; k < nz; k++) {
         idx_t j, i;
-        idx_t km = (k - 1 < 0) ? k + 1 : k - 1;
+        idx_t km = mir_m(k, nz);
         for (j = 0; j < ny; j++) {
-            idx_t jm = (j - 1 < 0) ? j + 1 : j - 1;
+            idx_t jm = mir_m(j, ny);
             for (i = 0; i < nx; i++) {
-                idx_t im = (i - 1 < 0) ? i + 1 : i - 1;
+                idx_t im = mir_m(i, nx);
                 idx_t c = sxy * k + nx * j + i;
                 REAL lv = lam[c * lam_is_arr];
                 REAL dv = (D1[c] - D1[sxy * k + nx * jm + i]) + (D2[c] - D2[sxy * k + nx * j + im]);
@@ -222,9 +228,9 @@ REAL FN(cr_rof)(const REAL *Input, REAL *Output, REAL *infovector, const REAL *l
     for (it = 0; it < iterationsNumb; it++) {
         int check = (epsil != 0) && (it % 5 == 0);
         if (check) copy_field(Output, prev, n);
-        rof_dir(Output, D1, dimX, dimY, dimZ, 0, ndim);
-        rof_dir(Output, D2, dimX, dimY, dimZ, 1, ndim);
-        if (ndim == 3) rof_dir(Output, D3, dimX, dimY, dimZ, 2, ndim);
+        rof_dir(Output, D1, dimX, dimY, dimZ, 0, ndim, ROF_EPS);
+        rof_dir(Output, D2, dimX, dimY, dimZ, 1, ndim, ROF_EPS);
+        if (ndim == 3) rof_dir(Output, D3, dimX, dimY, dimZ, 2, ndim, ROF_EPS);
         rof_update(D1, D2, D3, Output, Input, lambdaPar, lambda_is_arr, tau, dimX, dimY, dimZ, ndim);
         if (check && stop_check(&st, Output, prev, n, epsil)) break;
     }
@@ -841,6 +847,338 @@ REAL FN(cr_tgv)(const REAL *U0, REAL *U, REAL *infovector, REAL lambda, REAL alp
     for (a = 0; a < 3; a++) { free(P[a]); free(V[a]); free(Vold[a]); }
     for (a = 0; a < 6; a++) free(Q[a]);
     free(Uold);
+    infovector[0] = (REAL)it;
+    infovector[1] = st.re;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LLT-ROF: explicit descent on ROF-TV plus the Lysaker-Lundervold-Tai       */
+/* second-order term.  LLT_ROF_core.c:54-436                                 */
+/* ------------------------------------------------------------------------ */
+#define LLT_EPS 1.0e-12   /* double literals: LLT_ROF_core.c:21-22 */
+
+/* D_d = u_dd / (|u_dd| + 1e-12) along axis d, the denominator rounded from
+ * double (der2D_LLT / der3D_LLT, LLT_ROF_core.c:131-191). */
+static void llt_second(const REAL *u, REAL *D1, REAL *D2, REAL *D3, idx_t nx, idx_t ny, idx_t nz, int ndim)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        idx_t kp = mir_p(k, nz), km = mir_m(k, nz);
+        for (j = 0; j < ny; j++) {
+            idx_t jp = mir_p(j, ny), jm = mir_m(j, ny);
+            for (i = 0; i < nx; i++) {
+                idx_t c = sxy * k + nx * j + i;
+                REAL uc = u[c];
+                REAL dxx = u[sxy * k + nx * j + mir_p(i, nx)] - (REAL)2 * uc + u[sxy * k + nx * j + mir_m(i, nx)];
+                REAL dyy = u[sxy * k + nx * jp + i] - (REAL)2 * uc + u[sxy * k + nx * jm + i];
+                D1[c] = dxx / (REAL)(fabs((double)dxx) + LLT_EPS);
+                D2[c] = dyy / (REAL)(fabs((double)dyy) + LLT_EPS);
+                if (ndim == 3) {
+                    REAL dzz = u[sxy * kp + nx * j + i] - (REAL)2 * uc + u[sxy * km + nx * j + i];
+                    D3[c] = dzz / (REAL)(fabs((double)dzz) + LLT_EPS);
+                }
+            }
+        }
+    }
+}
+
+/* u += tau*(lambdaROF*div D_rof - lambdaLLT*lap D_llt - (u - f))
+ * (Update2D_LLT_ROF / Update3D_LLT_ROF, LLT_ROF_core.c:369-436). */
+static void llt_update(const REAL *const L[3], const REAL *const R[3], REAL *u, const REAL *f, REAL lam_rof,
+                       REAL lam_llt, REAL tau, idx_t nx, idx_t ny, idx_t nz, int ndim)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        idx_t kp = mir_p(k, nz), km = mir_m(k, nz);
+        for (j = 0; j < ny; j++) {
+            idx_t jp = mir_p(j, ny), jm = mir_m(j, ny);
+            for (i = 0; i < nx; i++) {
+                idx_t c = sxy * k + nx * j + i;
+                idx_t ip = sxy * k + nx * j + mir_p(i, nx), im = sxy * k + nx * j + mir_m(i, nx);
+                idx_t cjp = sxy * k + nx * jp + i, cjm = sxy * k + nx * jm + i;
+                REAL lap = (L[0][ip] - (REAL)2 * L[0][c] + L[0][im]) + (L[1][cjp] - (REAL)2 * L[1][c] + L[1][cjm]);
+                REAL dv = (R[0][c] - R[0][cjm]) + (R[1][c] - R[1][im]);
+                if (ndim == 3) {
+                    idx_t ckp = sxy * kp + nx * j + i, ckm = sxy * km + nx * j + i;
+                    lap = lap + (L[2][ckp] - (REAL)2 * L[2][c] + L[2][ckm]);
+                    dv = dv + (R[2][c] - R[2][ckm]);
+                }
+                u[c] += tau * (lam_rof * dv - lam_llt * lap - (u[c] - f[c]));
+            }
+        }
+    }
+}
+
+REAL FN(cr_llt)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaROF, REAL lambdaLLT, int iterationsNumb,
+                REAL tau, REAL epsil, int dimX, int dimY, int dimZ)
+{
+    int ndim = dimZ == 1 ? 2 : 3;
+    idx_t n = (idx_t)dimX * dimY * dimZ, alloc = n ? n : 1;
+    REAL *L[3], *R[3], *prev = NULL;
+    stop_state st = {0, 0};
+    int it, d;
+
+    for (d = 0; d < 3; d++) {
+        L[d] = calloc(alloc, sizeof(REAL));
+        R[d] = calloc(alloc, sizeof(REAL));
+    }
+    copy_field(Input, Output, n);
+    if (epsil != 0) prev = calloc(alloc, sizeof(REAL));
+    for (it = 0; it < iterationsNumb; it++) {
+        int check = (epsil != 0) && (it % 5 == 0);
+        if (check) copy_field(Output, prev, n);
+        /* the ROF fluxes of ROF-TV with EPS 1e-12 (LLT_ROF_core.c:198-363) */
+        rof_dir(Output, R[0], dimX, dimY, dimZ, 0, ndim, LLT_EPS);
+        rof_dir(Output, R[1], dimX, dimY, dimZ, 1, ndim, LLT_EPS);
+        if (ndim == 3) rof_dir(Output, R[2], dimX, dimY, dimZ, 2, ndim, LLT_EPS);
+        llt_second(Output, L[0], L[1], L[2], dimX, dimY, dimZ, ndim);
+        llt_update((const REAL *const *)L, (const REAL *const *)R, Output, Input, lambdaROF, lambdaLLT, tau, dimX,
+                   dimY, dimZ, ndim);
+        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+    }
+    for (d = 0; d < 3; d++) { free(L[d]); free(R[d]); }
+    free(prev);
+    infovector[0] = (REAL)it;
+    infovector[1] = st.re;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* NDF: linear and nonlinear (Huber, Perona-Malik, Tukey, threshold-linear,  */
+/* threshold-Huber) diffusion, explicit scheme.  Diffusion_core.c:53-413     */
+/* ------------------------------------------------------------------------ */
+
+/* The penalty's flux for a neighbour difference d (Diffusion_core.c:166-231). */
+static REAL ndf_flux(REAL d, REAL sigma, int penalty)
+{
+    switch (penalty) {
+    case 1:   /* Huber */
+        return fabs(d) > sigma ? (REAL)sgn_of(d) : d / sigma;
+    case 2: { /* Perona-Malik; powf(x, 2) is x*x */
+        REAL q = d / sigma;
+        return d / ((REAL)1 + q * q);
+    }
+    case 3:   /* Tukey biweight */
+        if (fabs(d) <= sigma) {
+            REAL q = d / sigma;
+            REAL t = (REAL)1 - q * q;
+            return d * (t * t);
+        }
+        return 0;
+    case 4:   /* threshold-constrained linear */
+        return fabs(d) > sigma ? (REAL)0 : d;
+    default:  /* 5: threshold-constrained Huber */
+        if (fabs(d) <= (REAL)2 * sigma) return fabs(d) > sigma ? (REAL)sgn_of(d) : d / sigma;
+        return 0;
+    }
+}
+
+/* One explicit step.  in_place = 0: every neighbour is read from the
+ * previous iterate (the synchronous explicit scheme).  in_place = 1: the
+ * reference's loop order run serially, each point updated in place so later
+ * points see earlier updates (Diffusion_core.c:108-413 on one thread). */
+static void ndf_step(const REAL *f, const REAL *uin, REAL *uout, REAL lambda, REAL sigma, REAL tau, int linear,
+                     int penalty, idx_t nx, idx_t ny, idx_t nz, int ndim, int in_place)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+    const REAL *src = in_place ? uout : uin;
+#pragma omp parallel for if (!in_place)
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        idx_t kp = mir_p(k, nz), km = mir_m(k, nz);
+        for (j = 0; j < ny; j++) {
+            idx_t jp = mir_p(j, ny), jm = mir_m(j, ny);
+            for (i = 0; i < nx; i++) {
+                idx_t c = sxy * k + nx * j + i;
+                REAL uc = src[c];
+                REAL d[6];
+                int nd = ndim == 3 ? 6 : 4, q;
+                REAL s;
+                d[0] = src[sxy * k + nx * j + mir_p(i, nx)] - uc;   /* e */
+                d[1] = src[sxy * k + nx * j + mir_m(i, nx)] - uc;   /* w */
+                d[2] = src[sxy * k + nx * jp + i] - uc;             /* n */
+                d[3] = src[sxy * k + nx * jm + i] - uc;             /* s */
+                if (ndim == 3) {
+                    d[4] = src[sxy * kp + nx * j + i] - uc;         /* u */
+                    d[5] = src[sxy * km + nx * j + i] - uc;         /* d */
+                }
+                if (!linear)
+                    for (q = 0; q < nd; q++) d[q] = ndf_flux(d[q], sigma, penalty);
+                s = d[0] + d[1] + d[2] + d[3];
+                if (ndim == 3) s = s + d[4] + d[5];
+                uout[c] = uc + tau * (lambda * s - (uc - f[c]));
+            }
+        }
+    }
+}
+
+static REAL ndf_run(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar, REAL sigmaPar,
+                    int iterationsNumb, REAL tau, int penaltytype, REAL epsil, int dimX, int dimY, int dimZ,
+                    int in_place)
+{
+    int ndim = dimZ == 1 ? 2 : 3;
+    idx_t n = (idx_t)dimX * dimY * dimZ, alloc = n ? n : 1;
+    /* sigma / sqrt(2) with a double square root and a double division
+     * (Diffusion_core.c:57) */
+    REAL sigma2 = (REAL)((double)sigmaPar / sqrt(2.0));
+    int linear = sigmaPar == 0;
+    REAL *prev = NULL, *tmp = in_place ? NULL : calloc(alloc, sizeof(REAL));
+    stop_state st = {0, 0};
+    int it;
+
+    copy_field(Input, Output, n);
+    if (epsil != 0) prev = calloc(alloc, sizeof(REAL));
+    for (it = 0; it < iterationsNumb; it++) {
+        int check = (epsil != 0) && (it % 5 == 0);
+        if (check) copy_field(Output, prev, n);
+        if (in_place) {
+            ndf_step(Input, Output, Output, lambdaPar, sigma2, tau, linear, penaltytype, dimX, dimY, dimZ, ndim, 1);
+        } else {
+            copy_field(Output, tmp, n);
+            ndf_step(Input, tmp, Output, lambdaPar, sigma2, tau, linear, penaltytype, dimX, dimY, dimZ, ndim, 0);
+        }
+        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+    }
+    free(prev);
+    free(tmp);
+    infovector[0] = (REAL)it;
+    infovector[1] = st.re;
+    return 0;
+}
+
+REAL FN(cr_ndf)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar, REAL sigmaPar, int iterationsNumb,
+                REAL tau, int penaltytype, REAL epsil, int dimX, int dimY, int dimZ)
+{
+    return ndf_run(Input, Output, infovector, lambdaPar, sigmaPar, iterationsNumb, tau, penaltytype, epsil, dimX, dimY,
+                   dimZ, 0);
+}
+
+REAL FN(cr_ndf_inplace)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar, REAL sigmaPar,
+                        int iterationsNumb, REAL tau, int penaltytype, REAL epsil, int dimX, int dimY, int dimZ)
+{
+    return ndf_run(Input, Output, infovector, lambdaPar, sigmaPar, iterationsNumb, tau, penaltytype, epsil, dimX, dimY,
+                   dimZ, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diff4th: anisotropic fourth-order diffusion (Hajiaboli), explicit scheme. */
+/* Diffus4th_order_core.c:44-265                                             */
+/* ------------------------------------------------------------------------ */
+#define D4_EPS 1.0e-7   /* a double literal: Diffus4th_order_core.c:23 */
+
+/* The weighted Laplacian W of u (Weighted_Laplc2D / 3D, :103-148, :176-235):
+ * central first and second differences with mirrored neighbours, the
+ * normal / tangential second derivatives divided by |grad|^2 (or, below
+ * 1e-7, by the double 1e-7), weighted by c = 1/(1 + |grad|^2/sigma^2). */
+static void d4_wlap(const REAL *u, REAL *W, REAL sigma2, idx_t nx, idx_t ny, idx_t nz, int ndim)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        idx_t k1 = mir_p(k, nz), k2 = mir_m(k, nz);
+        for (j = 0; j < ny; j++) {
+            idx_t j1 = mir_p(j, ny), j2 = mir_m(j, ny);
+            for (i = 0; i < nx; i++) {
+                idx_t i1 = mir_p(i, nx), i2 = mir_m(i, nx);
+                idx_t z0 = sxy * k;
+                idx_t c = z0 + nx * j + i;
+                REAL uc = u[c];
+                REAL gx = (REAL)0.5 * (u[z0 + nx * j + i2] - u[z0 + nx * j + i1]);
+                REAL gy = (REAL)0.5 * (u[z0 + nx * j2 + i] - u[z0 + nx * j1 + i]);
+                REAL gxs = gx * gx, gys = gy * gy;
+                REAL gxx = u[z0 + nx * j + i2] + u[z0 + nx * j + i1] - (REAL)2 * uc;
+                REAL gyy = u[z0 + nx * j2 + i] + u[z0 + nx * j1 + i] - (REAL)2 * uc;
+                REAL gxy = (REAL)0.25 * (u[z0 + nx * j2 + i2] + u[z0 + nx * j1 + i1] - u[z0 + nx * j1 + i2] -
+                                         u[z0 + nx * j2 + i1]);
+                REAL xy2 = (REAL)2 * gx * gy * gxy;
+                REAL den, nn, no, vn, vo, cw;
+                if (ndim == 3) {
+                    idx_t zp = sxy * k1, zm = sxy * k2;
+                    REAL gz = (REAL)0.5 * (u[zm + nx * j + i] - u[zp + nx * j + i]);
+                    REAL gzs = gz * gz;
+                    REAL gzz = u[zm + nx * j + i] + u[zp + nx * j + i] - (REAL)2 * uc;
+                    REAL gxz = (REAL)0.25 * (u[zm + nx * j + i2] - u[zm + nx * j + i1] - u[zp + nx * j + i2] +
+                                             u[zp + nx * j + i1]);
+                    REAL gyz = (REAL)0.25 * (u[zm + nx * j2 + i] - u[zm + nx * j1 + i] - u[zp + nx * j2 + i] +
+                                             u[zp + nx * j1 + i]);
+                    REAL xz1 = (REAL)2 * gx * gz * gxz, yz2 = (REAL)2 * gy * gz * gyz;
+                    den = gxs + gys + gzs;
+                    nn = gxx * gxs + gyy * gys + gzz * gzs + xy2 + xz1 + yz2;
+                    no = (gys + gzs) * gxx + (gxs + gzs) * gyy + (gxs + gys) * gzz - xy2 - xz1 - yz2;
+                } else {
+                    den = gxs + gys;
+                    nn = gxx * gxs + xy2 + gyy * gys;
+                    no = gxx * gys - xy2 + gyy * gxs;
+                }
+                if (den <= D4_EPS) {
+                    vn = (REAL)(nn / D4_EPS);
+                    vo = (REAL)(no / D4_EPS);
+                } else {
+                    vn = nn / den;
+                    vo = no / den;
+                }
+                cw = (REAL)1 / ((REAL)1 + den / sigma2);
+                W[c] = cw * cw * vn + cw * vo;
+            }
+        }
+    }
+}
+
+/* u += tau*(-lambda*lap W - (u - f)) (Diffusion_update_step2D / 3D, :150-172, :237-265). */
+static void d4_update(const REAL *W, REAL *u, const REAL *f, REAL lambda, REAL tau, idx_t nx, idx_t ny, idx_t nz,
+                      int ndim)
+{
+    idx_t k;
+    idx_t sxy = nx * ny;
+#pragma omp parallel for
+    for (k = 0; k < nz; k++) {
+        idx_t j, i;
+        idx_t k1 = mir_p(k, nz), k2 = mir_m(k, nz);
+        for (j = 0; j < ny; j++) {
+            idx_t j1 = mir_p(j, ny), j2 = mir_m(j, ny);
+            for (i = 0; i < nx; i++) {
+                idx_t z0 = sxy * k;
+                idx_t c = z0 + nx * j + i;
+                REAL s = (W[z0 + nx * j + mir_m(i, nx)] + W[z0 + nx * j + mir_p(i, nx)] - (REAL)2 * W[c]) +
+                         (W[z0 + nx * j2 + i] + W[z0 + nx * j1 + i] - (REAL)2 * W[c]);
+                if (ndim == 3) s = s + (W[sxy * k2 + nx * j + i] + W[sxy * k1 + nx * j + i] - (REAL)2 * W[c]);
+                u[c] += tau * (-lambda * s - (u[c] - f[c]));
+            }
+        }
+    }
+}
+
+REAL FN(cr_d4)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar, REAL sigmaPar, int iterationsNumb,
+               REAL tau, REAL epsil, int dimX, int dimY, int dimZ)
+{
+    int ndim = dimZ == 1 ? 2 : 3;
+    idx_t n = (idx_t)dimX * dimY * dimZ, alloc = n ? n : 1;
+    REAL sigma2 = sigmaPar * sigmaPar;
+    REAL *W = calloc(alloc, sizeof(REAL)), *prev = NULL;
+    stop_state st = {0, 0};
+    int it;
+
+    copy_field(Input, Output, n);
+    if (epsil != 0) prev = calloc(alloc, sizeof(REAL));
+    for (it = 0; it < iterationsNumb; it++) {
+        int check = (epsil != 0) && (it % 5 == 0);
+        if (check) copy_field(Output, prev, n);
+        d4_wlap(Output, W, sigma2, dimX, dimY, dimZ, ndim);
+        d4_update(W, Output, Input, lambdaPar, tau, dimX, dimY, dimZ, ndim);
+        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+    }
+    free(W);
+    free(prev);
     infovector[0] = (REAL)it;
     infovector[1] = st.re;
     return 0;

@@ -65,6 +65,10 @@ class _Lib:
             self.f_sb = getattr(self.lib, f"cr_sb_{prec}")
             self.f_tgv = getattr(self.lib, f"cr_tgv_{prec}")
             self.f_dtv = getattr(self.lib, f"cr_dtv_{prec}")
+            self.f_llt = getattr(self.lib, f"cr_llt_{prec}")
+            self.f_ndf = getattr(self.lib, f"cr_ndf_{prec}")
+            self.f_ndf_inplace = getattr(self.lib, f"cr_ndf_inplace_{prec}")
+            self.f_d4 = getattr(self.lib, f"cr_d4_{prec}")
         else:  # the reference's own symbol names (src/Core/regularisers_CPU/*.h)
             self.f_rof = self.lib.TV_ROF_CPU_main
             self.f_fgp = self.lib.TV_FGP_CPU_main
@@ -72,6 +76,12 @@ class _Lib:
             self.f_sb = self.lib.SB_TV_CPU_main
             self.f_tgv = self.lib.TGV_main
             self.f_dtv = self.lib.dTV_FGP_CPU_main
+            self.f_llt = self.lib.LLT_ROF_CPU_main
+            # the reference updates NDF in place inside its parallel loops;
+            # run on one thread that is the serial in-place scheme
+            self.f_ndf = None
+            self.f_ndf_inplace = self.lib.Diffusion_CPU_main
+            self.f_d4 = self.lib.Diffus4th_CPU_main
         i = ctypes.c_int
         self.f_rof.argtypes = [ptr, ptr, ptr, ptr, i, i, real, real, i, i, i]
         self.f_fgp.argtypes = [ptr, ptr, ptr, real, i, real, i, i, i, i, i]
@@ -79,8 +89,15 @@ class _Lib:
         self.f_sb.argtypes = [ptr, ptr, ptr, real, i, real, i, i, i, i]
         self.f_tgv.argtypes = [ptr, ptr, ptr, real, real, real, i, real, real, i, i, i]
         self.f_dtv.argtypes = [ptr, ptr, ptr, ptr, real, i, real, real, i, i, i, i, i]
-        for f in (self.f_rof, self.f_fgp, self.f_pd, self.f_sb, self.f_tgv, self.f_dtv):
-            f.restype = real
+        self.f_llt.argtypes = [ptr, ptr, ptr, real, real, i, real, real, i, i, i]
+        for f in (self.f_ndf, self.f_ndf_inplace):
+            if f is not None:
+                f.argtypes = [ptr, ptr, ptr, real, real, i, real, i, real, i, i, i]
+        self.f_d4.argtypes = [ptr, ptr, ptr, real, real, i, real, real, i, i, i]
+        for f in (self.f_rof, self.f_fgp, self.f_pd, self.f_sb, self.f_tgv, self.f_dtv, self.f_llt, self.f_ndf,
+                  self.f_ndf_inplace, self.f_d4):
+            if f is not None:
+                f.restype = real
 
     # -- helpers --------------------------------------------------------
     def _prep(self, x, out, zero_out=True):
@@ -139,6 +156,30 @@ class _Lib:
         self.f_dtv(self._p(x), self._p(r), self._p(out), self._p(info), lam, iters, tol, eta, method, nonneg, *d)
         return out, info
 
+    def llt(self, x, lam_rof, lam_llt, iters, tau, tol, out=None):
+        x, out, info = self._prep(x, out)
+        self.f_llt(self._p(x), self._p(out), self._p(info), lam_rof, lam_llt, iters, tau, tol, *_dims(x))
+        return out, info
+
+    def ndf(self, x, lam, sigma, iters, tau, penalty, tol, out=None, in_place=False):
+        """The synchronous explicit scheme; ``in_place=True`` runs the serial
+        in-place order instead (the reference library's order on one thread)."""
+        f = self.f_ndf_inplace if in_place else self.f_ndf
+        if f is None:
+            raise ValueError("the reference library has only the in-place scheme")
+        x, out, info = self._prep(x, out)
+        if in_place and self.kind == "ref":
+            with _single_thread():
+                f(self._p(x), self._p(out), self._p(info), lam, sigma, iters, tau, penalty, tol, *_dims(x))
+        else:
+            f(self._p(x), self._p(out), self._p(info), lam, sigma, iters, tau, penalty, tol, *_dims(x))
+        return out, info
+
+    def d4(self, x, lam, sigma, iters, tau, tol, out=None):
+        x, out, info = self._prep(x, out)
+        self.f_d4(self._p(x), self._p(out), self._p(info), lam, sigma, iters, tau, tol, *_dims(x))
+        return out, info
+
     def run(self, method, x, params, out=None):
         """Dispatch by method name with a parameter dict (keys as in the
         reference Python API)."""
@@ -153,6 +194,13 @@ class _Lib:
             return self.sb(x, p["mu"], p["iters"], p["tol"], p["method"], out=out)
         if method == "tgv":
             return self.tgv(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], out=out)
+        if method == "llt":
+            return self.llt(x, p["lambda_rof"], p["lambda_llt"], p["iters"], p["tau"], p["tol"], out=out)
+        if method == "ndf":
+            return self.ndf(x, p["lambda"], p["sigma"], p["iters"], p["tau"], p["penalty"], p["tol"], out=out,
+                            in_place=p.get("in_place", False))
+        if method == "d4":
+            return self.d4(x, p["lambda"], p["sigma"], p["iters"], p["tau"], p["tol"], out=out)
         if method == "dtv":   # p["ref"]: the reference image, an array of x's shape
             return self.dtv(x, p["ref"], p["lambda"], p["iters"], p["tol"], p["eta"], p["method"], p["nonneg"],
                             out=out)
@@ -160,6 +208,18 @@ class _Lib:
 
 
 _cache = {}
+
+
+class _single_thread:
+    """OpenMP on one thread for the duration of a call (libgomp's omp_set_num_threads)."""
+
+    def __enter__(self):
+        self.gomp = ctypes.CDLL("libgomp.so.1")
+        self.prev = self.gomp.omp_get_max_threads()
+        self.gomp.omp_set_num_threads(1)
+
+    def __exit__(self, *exc):
+        self.gomp.omp_set_num_threads(self.prev)
 
 
 def cpu_ref(prec="f32"):

@@ -11,6 +11,10 @@ src/Core/regularisers_CPU/*.c) and stores the input, the initial output buffer
 (FGP's first residual reads it), the reference output and the reference
 infovector in ``ref_cases.npz``, with the case table in ``ref_cases.json``.
 
+NDF fixtures are the reference library's in-place scheme run on one thread
+(its serial order); the reference's parallel loops update in place, so with
+more threads its result depends on scheduling.
+
 Inputs are small, non-cubic and seeded: a 2D 37x53 and 3D 11x13x17 piecewise
 constant phantom plus noise, a 3D 24x40x56 volume, and a 96x96 crop of the
 reference's own test image (test/test_imageLena.bin, scaled by 1/255 as in
@@ -55,6 +59,9 @@ def inputs():
     rng2 = np.random.Generator(np.random.PCG64(24))
     noise2 = phantoms.gaussian_noise(rng2, crop.size, 1.0).reshape(crop.shape)
     ins["lena96_ref"] = (crop + 0.01 * crop * noise2).astype(np.float32)
+    # noise-free phantoms: flat regions reach Diff4th's |grad|^2 <= 1e-7 branch
+    ins["p2d_clean"] = phantoms.shapes_2d((37, 53), 8, seed=11)
+    ins["p3d_clean"] = phantoms.ellipsoids_3d((11, 13, 17), 8, seed=13)
     # a non-zero initial Output for FGP's first residual (FGP_TV_core.c:66,123)
     ins["p2d_init"] = (0.5 * ins["p2d"]).astype(np.float32)
     ins["p3d_init"] = (0.5 * ins["p3d"]).astype(np.float32)
@@ -108,6 +115,26 @@ def cases():
     # the reference's dTV test parameters (test_2d_cpu_vs_gpu.py:556-566), fewer iterations
     cs.append(("dtv", lena, {"lambda": 0.02, "iters": 300, "tol": 0.0, "eta": 0.2, "method": 0, "nonneg": 0,
                              "ref_input": lena + "_ref"}))
+    # explicit diffusion family (appended likewise)
+    for inp in ("p2d", "p3d"):
+        cs.append(("llt", inp, {"lambda_rof": 0.01, "lambda_llt": 0.0085, "iters": 60, "tau": 0.002, "tol": 0.0}))
+        cs.append(("llt", inp, {"lambda_rof": 0.04, "lambda_llt": 0.02, "iters": 300, "tau": 0.01, "tol": 2e-3}))
+        cs.append(("ndf", inp, {"lambda": 0.02, "sigma": 0.0, "iters": 40, "tau": 0.01, "penalty": 1, "tol": 0.0}))
+        for pen in (1, 2, 3, 4, 5):
+            cs.append(("ndf", inp, {"lambda": 0.02, "sigma": 0.05, "iters": 40, "tau": 0.01, "penalty": pen,
+                                    "tol": 0.0}))
+        cs.append(("ndf", inp, {"lambda": 0.05, "sigma": 0.05, "iters": 400, "tau": 0.02, "penalty": 2,
+                                "tol": 1e-3}))
+        cs.append(("d4", inp, {"lambda": 0.8, "sigma": 0.02, "iters": 40, "tau": 1e-4, "tol": 0.0}))
+        cs.append(("d4", inp, {"lambda": 2.0, "sigma": 0.1, "iters": 300, "tau": 0.005, "tol": 2e-3}))
+        cs.append(("d4", inp + "_clean", {"lambda": 0.8, "sigma": 0.02, "iters": 20, "tau": 1e-4, "tol": 0.0}))
+    cs.append(("llt", big, {"lambda_rof": 0.02, "lambda_llt": 0.01, "iters": 40, "tau": 0.005, "tol": 0.0}))
+    cs.append(("ndf", big, {"lambda": 0.02, "sigma": 0.05, "iters": 40, "tau": 0.01, "penalty": 1, "tol": 0.0}))
+    cs.append(("d4", big, {"lambda": 0.8, "sigma": 0.02, "iters": 20, "tau": 1e-4, "tol": 0.0}))
+    # the reference's 2D CPU-vs-GPU test parameters (test_2d_cpu_vs_gpu.py:370-378, :431-440, :496-504)
+    cs.append(("llt", lena, {"lambda_rof": 0.01, "lambda_llt": 0.0085, "iters": 1000, "tau": 1e-4, "tol": 0.0}))
+    cs.append(("ndf", lena, {"lambda": 0.02, "sigma": 0.017, "iters": 1500, "tau": 0.01, "penalty": 1, "tol": 0.0}))
+    cs.append(("d4", lena, {"lambda": 0.8, "sigma": 0.02, "iters": 1000, "tau": 1e-4, "tol": 0.0}))
     return cs
 
 
@@ -123,8 +150,13 @@ def main():
         init_key = p.pop("out_init", None)
         ref_key = p.pop("ref_input", None)
         out = ins[init_key].copy() if init_key else None
-        run_p = dict(p, ref=ins[ref_key]) if ref_key else p
+        run_p = dict(p, ref=ins[ref_key]) if ref_key else dict(p)
+        if method == "ndf":
+            # the reference updates in place inside its parallel loops: its
+            # deterministic reading is the serial order (one thread)
+            run_p["in_place"] = True
         y, info = ref.run(method, ins[inp], run_p, out=out)
+        assert np.all(np.isfinite(y)), (method, inp, p)
         arrays[f"c{idx}_out"] = y
         arrays[f"c{idx}_info"] = info
         row = {"id": idx, "method": method, "input": inp, "params": p, "out_init": init_key}

@@ -24,7 +24,10 @@ def test_oracle_matches_reference_fixture(case):
     g = golden()
     x = g.input(case)
     out0 = g.out_init(case)
-    y, info = oracle.cpu_ref().run(case["method"], x, g.params(case), out=out0)
+    p = g.params(case)
+    if case["method"] == "ndf":
+        p["in_place"] = True   # the fixture holds the reference's serial in-place order
+    y, info = oracle.cpu_ref().run(case["method"], x, p, out=out0)
     y_ref, info_ref = g.expected(case)
     assert bits_equal(y, y_ref), f"max |diff| = {np.abs(y - y_ref).max()}"
     assert bits_equal(info, info_ref), f"infovector {info} vs reference {info_ref}"
@@ -38,6 +41,11 @@ def _ref_or_skip():
 
 
 EXTRA = [
+    ("llt", {"lambda_rof": 0.03, "lambda_llt": 0.01, "iters": 25, "tau": 0.004, "tol": 0.0}),
+    ("llt", {"lambda_rof": 0.01, "lambda_llt": 0.03, "iters": 120, "tau": 0.01, "tol": 5e-3}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 25, "tau": 0.02, "penalty": 3, "tol": 0.0, "in_place": True}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.2, "iters": 60, "tau": 0.02, "penalty": 5, "tol": 5e-3, "in_place": True}),
+    ("d4", {"lambda": 1.0, "sigma": 0.05, "iters": 25, "tau": 0.001, "tol": 0.0}),
     ("rof", {"lambda": 0.03, "iters": 25, "tau": 0.002, "tol": 0.0}),
     ("fgp", {"lambda": 0.07, "iters": 25, "tol": 0.0, "method": 0, "nonneg": 1}),
     ("fgp", {"lambda": 0.01, "iters": 120, "tol": 5e-3, "method": 1, "nonneg": 0}),
@@ -47,7 +55,7 @@ EXTRA = [
 ]
 
 
-@pytest.mark.parametrize("shape", [(29, 31), (9, 10, 23), (3, 5, 7)])
+@pytest.mark.parametrize("shape", [(29, 31), (9, 10, 23), (3, 5, 7), (2, 3), (2, 2, 2)])
 @pytest.mark.parametrize("method,params", EXTRA, ids=[f"{m}{i}" for i, (m, _) in enumerate(EXTRA)])
 def test_oracle_matches_reference_library(shape, method, params):
     ref = _ref_or_skip()

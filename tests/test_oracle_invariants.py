@@ -19,6 +19,20 @@ reference fixtures.
   with a zero reference image the gradient field B is zero, no projection
   happens, and 2D dTV equals 2D FGP (same 1/(8 lambda) step) up to the
   rounding of t_k, which dTV takes through a double square root.
+* NDF, synchronous explicit scheme: with sigma = 0 (linear diffusion) a
+  product of cosines cos(pi a i/(nx-1)) cos(pi b j/(ny-1)) is an eigenvector
+  of the mirrored Laplacian, so from u = f = v every iterate is s_n v with
+  s_{n+1} = s_n + tau (lambda mu s_n - (s_n - 1)); the oracle must follow that
+  closed form (a wrong boundary mirror, a swapped tau / lambda or an in-place
+  update breaks it).  The threshold-linear penalty with a threshold above
+  every difference is linear diffusion bit for bit.
+* Diff4th: constant images are fixed points bit for bit; scaling f by 2 and
+  sigma by 2 scales the output by 2 bit for bit over a few iterations (while
+  no |grad|^2 reaches the fixed 1e-7 floor, which does not scale).
+* LLT-ROF (on smooth ramps, whose differences stay far above the flux
+  floors): with lambda_LLT = 0 it is ROF-TV with the floor 1e-12 instead of
+  1e-8, within 1e-6 of the ROF oracle; scaling f and both lambdas by 2
+  scales the output by 2 within 1e-6.
 * The FP64 build agrees with the FP32 build to FP32 accuracy.
 """
 import numpy as np
@@ -155,3 +169,84 @@ def test_dtv_zero_reference_is_fgp_2d(method):
     err = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
     assert err < 1e-6, err
     assert not np.allclose(a, F2, atol=1e-3)   # and it did smooth
+
+
+def _eig2d(nx, ny, a, b):
+    i = np.arange(nx)
+    j = np.arange(ny)
+    vx = np.cos(np.pi * a * i / (nx - 1))
+    vy = np.cos(np.pi * b * j / (ny - 1))
+    mu = -(2 - 2 * np.cos(np.pi * a / (nx - 1))) - (2 - 2 * np.cos(np.pi * b / (ny - 1)))
+    return np.outer(vy, vx), mu
+
+
+@pytest.mark.parametrize("a,b", [(1, 0), (2, 3), (5, 1)])
+def test_ndf_linear_eigenvector_closed_form(a, b):
+    lib = oracle.cpu_ref()
+    v, mu = _eig2d(40, 33, a, b)
+    f = v.astype(np.float32)
+    lam, tau, n = 1.0, 0.1, 30
+    u, _ = lib.ndf(f, lam, 0.0, n, tau, 1, 0.0)
+    s = 1.0
+    for _ in range(n):
+        s = s + tau * (lam * mu * s - (s - 1.0))
+    assert np.abs(u - s * v).max() < 2e-6, np.abs(u - s * v).max()
+    # the serial in-place order is a different scheme: it leaves the closed form
+    w, _ = lib.ndf(f, lam, 0.0, n, tau, 1, 0.0, in_place=True)
+    assert np.abs(w - s * v).max() > 2e-5
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+def test_ndf_threshold_linear_above_all_differences_is_linear(f):
+    lib = oracle.cpu_ref()
+    a, _ = lib.ndf(f, 0.03, 0.0, 30, 0.02, 1, 0.0)
+    b, _ = lib.ndf(f, 0.03, 10.0, 30, 0.02, 4, 0.0)
+    assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(17, 19), (5, 6, 7)], ids=["2d", "3d"])
+def test_d4_constant_image_is_fixed(shape):
+    lib = oracle.cpu_ref()
+    x = np.full(shape, 0.37, dtype=np.float32)
+    y, _ = lib.d4(x, 0.8, 0.02, 20, 1e-3, 0.0)
+    assert bits_equal(y, x)
+
+
+@pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
+def test_d4_scale_equivariance(f):
+    # a few iterations: on the mirrored edges two central differences are 0,
+    # and later the third can fall under the fixed 1e-7 floor in one run and
+    # not in the scaled one
+    lib = oracle.cpu_ref()
+    a, _ = lib.d4(f, 1.0, 0.05, 5, 1e-3, 0.0)
+    b, _ = lib.d4(2 * f, 1.0, 0.1, 5, 1e-3, 0.0)
+    assert bits_equal(b, 2 * a)
+
+
+def _ramp(shape):
+    # differences bounded away from 0, so the 1e-12 / 1e-8 flux floors stay negligible
+    g = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    v = sum((0.05 + 0.02 * d) * g[d] for d in range(len(shape))) + 0.01 * np.sin(0.7 * g[-1]) * np.cos(0.3 * g[0])
+    return v.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(23, 31), (7, 9, 13)], ids=["2d", "3d"])
+def test_llt_without_second_order_term_is_rof(shape):
+    lib = oracle.cpu_ref()
+    f = _ramp(shape)
+    a, _ = lib.llt(f, 0.03, 0.0, 40, 0.002, 0.0)
+    b, _ = lib.rof(f, 0.03, 40, 0.002, 0.0)
+    err = np.abs(a.astype(np.float64) - b).max()
+    assert err < 1e-6, err
+    assert np.abs(a - f).max() > 1e-3   # and it did move
+
+
+@pytest.mark.parametrize("shape", [(23, 31), (7, 9, 13)], ids=["2d", "3d"])
+def test_llt_scale_equivariance(shape):
+    lib = oracle.cpu_ref()
+    f = _ramp(shape)
+    a, _ = lib.llt(f, 0.02, 0.01, 30, 0.005, 0.0)
+    b, _ = lib.llt(2 * f, 0.04, 0.02, 30, 0.005, 0.0)
+    err = np.abs(b.astype(np.float64) - 2 * a).max()
+    assert err < 1e-6, err
+    assert np.abs(a - f).max() > 1e-3
