@@ -29,7 +29,7 @@ Also reported:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu] [--no-batch]
                        [--no-serve] [--serve-window SECONDS]
        python bench.py --profile-pass --steps K --warmup 0   (the headline solves only)
-       python bench.py --workload {tgv3d,pd3d,dtv3d,copy,batch,serve} --steps K   (single-workload runs for
+       python bench.py --workload {tgv3d,pd3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
        profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
        known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -178,6 +178,8 @@ def secondary(D, torch):
     # FGP-dTV 3D, guided by the clean phantom with 1% noise
     log("bench: FGP-dTV 3D 256^3")
     out["fgp_dtv_3d_256_fp32"] = dtv_section(D, torch, x)
+    log("bench: explicit diffusion 3D 256^3")
+    out.update(diffusion_section(D, torch, x))
     del x
     # ROF 2D
     log("bench: ROF-TV 2D 512^2")
@@ -216,6 +218,44 @@ def dtv_section(D, torch, x):
             "alg_bytes_per_voxel_iter": 52,
             "kernel_alg_GBps": 52 * n / (m * 1e-3) / 1e9 if m else None,
             "kernel_frac_of_8TBps": 52 * n / (m * 1e-3) / 1e9 / HBM_PEAK_GBS if m else None}
+
+
+DIFFUSION = {   # the reference tests' parameters (test_2d_cpu_vs_gpu.py:370-378, :431-440, :496-504)
+    "llt": ((0.01, 0.0085), {"tau": 1e-4}, "llt_iter_kernel<3,8,uint32>"),
+    "ndf": ((0.02, 0.017), {"tau": 0.01, "penalty": 1}, "ndf_iter_kernel<3,8,uint32>"),
+    "d4": ((0.8, 0.02), {"tau": 1e-4}, "d4_iter_kernel<3,8,uint32>"),
+}
+
+
+def run_diffusion(D, name, x, iters, tol=0.0, profile=False, check_every=5):
+    (a, b), kw, _ = DIFFUSION[name]
+    if name == "llt":
+        return D.llt(x, a, b, iters, kw["tau"], tol, profile=profile, check_every=check_every)
+    if name == "ndf":
+        return D.ndf(x, a, b, iters, kw["tau"], kw["penalty"], tol, profile=profile, check_every=check_every)
+    return D.d4(x, a, b, iters, kw["tau"], tol, profile=profile, check_every=check_every)
+
+
+def diffusion_section(D, torch, x):
+    """LLT-ROF, NDF (Huber) and Diff4th on 256^3, 500 iterations each: whole-solve
+    rate and kernel time per iteration.  12 B/voxel-iteration of HBM traffic;
+    the divisions and square roots of every flux bound these kernels."""
+    n = x.numel()
+    out = {}
+    for name in ("llt", "ndf", "d4"):
+        run_diffusion(D, name, x, 10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, info = run_diffusion(D, name, x, 500)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        run_diffusion(D, name, x, 50, profile=True)
+        m, _, c = D.last_profile()
+        out[f"{name}_3d_256_fp32"] = {
+            "voxel_iters_per_s": n * info[0] / dt, "iterations": int(info[0]), "seconds": round(dt, 4),
+            "kernel": DIFFUSION[name][2], "kernel_ms_mean": m, "iterations_timed": c,
+            "alg_bytes_per_voxel_iter": 12, "kernel_alg_GBps": 12 * n / (m * 1e-3) / 1e9 if m else None}
+    return out
 
 
 def batched_jobs(seed=4):
@@ -422,6 +462,12 @@ def single_workload(name, steps, D, torch):
         D.pd(x, 0.02, steps, 0.0, 12.0, 0, 1, profile=True)
         m, _, c = D.last_profile()
         return {"workload": "PD-TV 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c}
+    if name in ("llt3d", "ndf3d", "d43d"):
+        x = torch.from_numpy(fgp_workload()).cuda()
+        run_diffusion(D, name[:-2], x, steps, profile=True)
+        m, _, c = D.last_profile()
+        return {"workload": f"{name[:-2]} 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c,
+                "voxel_iters_per_s_kernel": x.numel() / (m * 1e-3) if m else None}
     if name == "dtv3d":
         x = torch.from_numpy(fgp_workload()).cuda()
         r = torch.from_numpy(dtv_reference()).cuda()
@@ -446,8 +492,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-window", type=float, default=0.5,
                     help="seconds of Poisson arrivals replayed by the served section")
-    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "dtv3d", "copy", "batch",
-                                                                "serve"])
+    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "dtv3d", "llt3d", "ndf3d",
+                                                                "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

@@ -144,6 +144,19 @@ def TGV_GPU(inputData, lambdaPar, alpha1, alpha0, iterationsNumb, L2, epsil, gpu
     return out
 
 
+def LLT_ROF_GPU(inputData, lambdaROF, lambdaLLT, iterationsNumb, tau, epsil, gpu_device, out=None, infovector=None):
+    lib = require_gpu_lib()
+    lib.LLT_ROF_GPU_main.argtypes = [_fp, _fp, _fp, _f, _f, _i, _f, _f, _i, _i, _i, _i]
+    lib.LLT_ROF_GPU_main.restype = _i
+    x = _f32(inputData)
+    out = _out(x, out)
+    infovector = _info(infovector)
+    rc = lib.LLT_ROF_GPU_main(_p(x), _p(out), _p(infovector), lambdaROF, lambdaLLT, iterationsNumb, tau, epsil,
+                              _gpu_device(gpu_device), *_dims(x))
+    _check(rc, "LLT_ROF_GPU_main")
+    return out
+
+
 def _ref_image(x, inputRef):
     r = _f32(inputRef)
     if r.shape != x.shape:
@@ -249,4 +262,15 @@ def dTV_FGP_CPU(inputData, inputRef, lambdaPar, iterationsNumb, epsil, eta, meth
     infovector = _info(infovector)
     lib.dTV_FGP_CPU_main(_p(x), _p(r), _p(out), _p(infovector), lambdaPar, iterationsNumb, epsil, eta, methodTV,
                          nonneg, *_dims(x))
+    return out
+
+
+def LLT_ROF_CPU(inputData, lambdaROF, lambdaLLT, iterationsNumb, tau, epsil, out=None, infovector=None):
+    lib = _require_cpu_lib()
+    lib.LLT_ROF_CPU_main.argtypes = [_fp, _fp, _fp, _f, _f, _i, _f, _f, _i, _i, _i]
+    lib.LLT_ROF_CPU_main.restype = _f
+    x = _f32(inputData)
+    out = _out(x, out)
+    infovector = _info(infovector)
+    lib.LLT_ROF_CPU_main(_p(x), _p(out), _p(infovector), lambdaROF, lambdaLLT, iterationsNumb, tau, epsil, *_dims(x))
     return out

@@ -94,6 +94,9 @@ _SIGS = {
     "cilreg_pd_dev": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_sb_dev": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_llt_dev": [_vp, _vp, _vp, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_ndf_dev": [_vp, _vp, _vp, _f, _f, _i, _f, _i, _f, _i, _i, _i, _vp, _vp],
+    "cilreg_d4_dev": [_vp, _vp, _vp, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
     "cilreg_dtv_dev": [_vp, _vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev_f64": [_vp, _vp, _vp, _d, _d, _d, _i, _d, _d, _i, _i, _i, _vp, _vp],
@@ -173,6 +176,24 @@ def tgv(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
     import torch
     out = _prep(x, out, torch.float32)
     return _run("cilreg_tgv_dev", x, out, torch.float32, [lam, alpha1, alpha0, iters, L2, tol], make_opts(**kw))
+
+
+def llt(x, lam_rof, lam_llt, iters, tau, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_llt_dev", x, out, torch.float32, [lam_rof, lam_llt, iters, tau, tol], make_opts(**kw))
+
+
+def ndf(x, lam, sigma, iters, tau, penalty=1, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_ndf_dev", x, out, torch.float32, [lam, sigma, iters, tau, penalty, tol], make_opts(**kw))
+
+
+def d4(x, lam, sigma, iters, tau, tol=0.0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float32)
+    return _run("cilreg_d4_dev", x, out, torch.float32, [lam, sigma, iters, tau, tol], make_opts(**kw))
 
 
 def dtv(x, ref, lam, iters, tol=0.0, eta=0.01, method=0, nonneg=0, out=None, **kw):
@@ -295,6 +316,12 @@ def run(method, x, params, out=None, **kw):
         return sb(x, p["mu"], p["iters"], p["tol"], p["method"], out=out, **kw)
     if method == "tgv":
         return tgv(x, p["lambda"], p["alpha1"], p["alpha0"], p["iters"], p["L2"], p["tol"], out=out, **kw)
+    if method == "llt":
+        return llt(x, p["lambda_rof"], p["lambda_llt"], p["iters"], p["tau"], p["tol"], out=out, **kw)
+    if method == "ndf":
+        return ndf(x, p["lambda"], p["sigma"], p["iters"], p["tau"], p["penalty"], p["tol"], out=out, **kw)
+    if method == "d4":
+        return d4(x, p["lambda"], p["sigma"], p["iters"], p["tau"], p["tol"], out=out, **kw)
     if method == "dtv":   # p["ref"]: the reference image, a tensor of x's shape
         return dtv(x, p["ref"], p["lambda"], p["iters"], p["tol"], p["eta"], p["method"], p["nonneg"], out=out, **kw)
     raise ValueError(method)

@@ -1,6 +1,7 @@
-"""User API: ``ROF_TV, FGP_TV, PD_TV, SB_TV, TGV, FGP_dTV`` with ``device=`` dispatch.
+"""User API: ``ROF_TV, FGP_TV, PD_TV, SB_TV, TGV, FGP_dTV, LLT_ROF, NDF, Diff4th``
+with ``device=`` dispatch.
 
-Mirrors src/Python/ccpi/filters/regularisers.py:19-49 (FGP_dTV: :44) of the reference:
+Mirrors src/Python/ccpi/filters/regularisers.py:19-49 (FGP_dTV :44, LLT_ROF :42, NDF :46, Diff4th :47) of the reference:
 ``device='cpu'`` calls the C-OMP function, ``'gpu'`` or an integer device
 index calls the MI355X function with ``gpu_device`` set, anything else raises
 KeyError.  Positional argument order per method:
@@ -11,6 +12,9 @@ KeyError.  Positional argument order per method:
 * ``SB_TV(x, mu, iterations, tol, methodTV)``
 * ``TGV(x, lambda, alpha1, alpha0, iterations, L2, tol)``
 * ``FGP_dTV(x, x_ref, lambda, iterations, tol, eta, methodTV, nonneg)``
+* ``LLT_ROF(x, lambdaROF, lambdaLLT, iterations, tau, tol)``
+* ``NDF(x, lambda, sigma, iterations, tau, penalty_type, tol)``
+* ``Diff4th(x, lambda, sigma, iterations, tau, tol)``
 
 each also taking ``out=`` and ``infovector=`` (``Output=`` for PD on the GPU,
 as in the reference, TV.py:567).
@@ -24,6 +28,9 @@ from .TV import PDTV_CPU, PDTV_GPU
 from .TV import SB_TV_CPU, SB_TV_GPU
 from .TV import TGV_CPU, TGV_GPU
 from .TV import dTV_FGP_CPU, dTV_FGP_GPU
+from .TV import LLT_ROF_CPU, LLT_ROF_GPU
+from .diffusion import NDF_CPU, NDF_GPU
+from .diffusion import Diffus4th_CPU, Diffus4th_GPU
 from .utils import cilregcuda
 
 
@@ -44,3 +51,6 @@ PD_TV = create_wrapper(PDTV_CPU, PDTV_GPU)
 SB_TV = create_wrapper(SB_TV_CPU, SB_TV_GPU)
 TGV = create_wrapper(TGV_CPU, TGV_GPU)
 FGP_dTV = create_wrapper(dTV_FGP_CPU, dTV_FGP_GPU)
+LLT_ROF = create_wrapper(LLT_ROF_CPU, LLT_ROF_GPU)
+NDF = create_wrapper(NDF_CPU, NDF_GPU)
+Diff4th = create_wrapper(Diffus4th_CPU, Diffus4th_GPU)

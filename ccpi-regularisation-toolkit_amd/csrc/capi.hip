@@ -197,6 +197,46 @@ int dTV_FGP_GPU_main(float* Input, float* InputRef, float* Output, float* infove
                          });
 }
 
+int LLT_ROF_GPU_main(float* Input, float* Output, float* infovector, float lambdaROF, float lambdaLLT,
+                     int iterationsNumb, float tau, float epsil, int gpu_device, int N, int M, int Z) {
+    if (Z == 0) Z = 1;   // LLT_ROF_GPU_core.cu:469
+    if (!dims_ok(N, M, Z)) return -1;
+    const size_t n = (size_t)N * M * Z;
+    Opts o;
+    return host_call(gpu_device, Input, Output, infovector, n, false, false,
+                     [&](const float* di, float* dout, hipStream_t s, int* done, float* re) {
+                         return llt_solve(di, dout, lambdaROF, lambdaLLT, iterationsNumb, tau, epsil, N, M, Z, o, s,
+                                          done, re);
+                     });
+}
+
+int NonlDiff_GPU_main(float* Input, float* Output, float* infovector, float lambdaPar, float sigmaPar,
+                      int iterationsNumb, float tau, int penaltytype, float epsil, int gpu_device, int N, int M,
+                      int Z) {
+    if (Z == 0) Z = 1;
+    if (!dims_ok(N, M, Z)) return -1;
+    const size_t n = (size_t)N * M * Z;
+    Opts o;
+    return host_call(gpu_device, Input, Output, infovector, n, false, false,
+                     [&](const float* di, float* dout, hipStream_t s, int* done, float* re) {
+                         return ndf_solve(di, dout, lambdaPar, sigmaPar, iterationsNumb, tau, penaltytype, epsil, N,
+                                          M, Z, o, s, done, re);
+                     });
+}
+
+int Diffus4th_GPU_main(float* Input, float* Output, float* infovector, float lambdaPar, float sigmaPar,
+                       int iterationsNumb, float tau, float epsil, int gpu_device, int N, int M, int Z) {
+    if (Z == 0) Z = 1;
+    if (!dims_ok(N, M, Z)) return -1;
+    const size_t n = (size_t)N * M * Z;
+    Opts o;
+    return host_call(gpu_device, Input, Output, infovector, n, false, false,
+                     [&](const float* di, float* dout, hipStream_t s, int* done, float* re) {
+                         return d4_solve(di, dout, lambdaPar, sigmaPar, iterationsNumb, tau, epsil, N, M, Z, o, s,
+                                         done, re);
+                     });
+}
+
 // ---------------------------------------------------------------------------
 // device-resident extensions
 // ---------------------------------------------------------------------------
@@ -279,6 +319,33 @@ int cilreg_tgv_dev(const float* d_input, float* d_output, float* infovector, flo
     else
         rc = tgv_solve<float, float>(d_input, d_output, lambda, alpha1, alpha0, iter, L2, epsil, dimX, dimY, dimZ, o,
                                      s, &done, &re);
+    DEV_EPILOGUE()
+}
+
+int cilreg_llt_dev(const float* d_input, float* d_output, float* infovector, float lambdaROF, float lambdaLLT,
+                   int iter, float tau, float epsil, int dimX, int dimY, int dimZ, const cilreg_opts* opts,
+                   void* stream) {
+    if (dimZ == 0) dimZ = 1;
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    rc = llt_solve(d_input, d_output, lambdaROF, lambdaLLT, iter, tau, epsil, dimX, dimY, dimZ, o, s, &done, &re);
+    DEV_EPILOGUE()
+}
+
+int cilreg_ndf_dev(const float* d_input, float* d_output, float* infovector, float lambdaPar, float sigmaPar, int iter,
+                   float tau, int penaltytype, float epsil, int dimX, int dimY, int dimZ, const cilreg_opts* opts,
+                   void* stream) {
+    if (dimZ == 0) dimZ = 1;
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    rc = ndf_solve(d_input, d_output, lambdaPar, sigmaPar, iter, tau, penaltytype, epsil, dimX, dimY, dimZ, o, s,
+                   &done, &re);
+    DEV_EPILOGUE()
+}
+
+int cilreg_d4_dev(const float* d_input, float* d_output, float* infovector, float lambdaPar, float sigmaPar, int iter,
+                  float tau, float epsil, int dimX, int dimY, int dimZ, const cilreg_opts* opts, void* stream) {
+    if (dimZ == 0) dimZ = 1;
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    rc = d4_solve(d_input, d_output, lambdaPar, sigmaPar, iter, tau, epsil, dimX, dimY, dimZ, o, s, &done, &re);
     DEV_EPILOGUE()
 }
 

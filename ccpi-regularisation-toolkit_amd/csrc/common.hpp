@@ -233,10 +233,11 @@ __device__ inline bool solve_stopped(const Ctrl* c) { return c != nullptr && c->
 // ---------------------------------------------------------------------------
 template <typename T> struct Mixed;
 template <> struct Mixed<float> {
-    // sqrtf((float)((double)s + 1e-8)): ROF_TV_core.c:160,199,223,262
-    __device__ static inline float rof_sqrt_f(float s) { return sqrtf((float)((double)s + 1.0e-8)); }
-    // (float)sqrt((double)s + 1e-8): the 3D D1, ROF_TV_core.c:136
-    __device__ static inline float rof_sqrt_d(float s) { return (float)sqrt((double)s + 1.0e-8); }
+    // sqrtf((float)((double)s + eps)): ROF_TV_core.c:160,199,223,262 (eps 1e-8;
+    // LLT_ROF_core.c:256,296,320,359 with 1e-12)
+    __device__ static inline float rof_sqrt_f(float s, double eps = 1.0e-8) { return sqrtf((float)((double)s + eps)); }
+    // (float)sqrt((double)s + eps): the 3D D1, ROF_TV_core.c:136, LLT_ROF_core.c:232
+    __device__ static inline float rof_sqrt_d(float s, double eps = 1.0e-8) { return (float)sqrt((double)s + eps); }
     // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
     __device__ static inline float pd_div(float num, float lt) { return (float)((double)num / (1.0 + (double)lt)); }
     // squared minmod, ROF_TV_core.c:132-133.  The reference multiplies in
@@ -252,8 +253,8 @@ template <> struct Mixed<float> {
     }
 };
 template <> struct Mixed<double> {
-    __device__ static inline double rof_sqrt_f(double s) { return sqrt(s + 1.0e-8); }
-    __device__ static inline double rof_sqrt_d(double s) { return sqrt(s + 1.0e-8); }
+    __device__ static inline double rof_sqrt_f(double s, double eps = 1.0e-8) { return sqrt(s + eps); }
+    __device__ static inline double rof_sqrt_d(double s, double eps = 1.0e-8) { return sqrt(s + eps); }
     __device__ static inline double pd_div(double num, double lt) { return num / (1.0 + lt); }
     __device__ static inline double minmod_sq(double a, double b) {
         int sa = (a > 0.0) - (a < 0.0), sb = (b > 0.0) - (b < 0.0);

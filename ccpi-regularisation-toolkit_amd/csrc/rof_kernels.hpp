@@ -25,21 +25,24 @@ __device__ inline int mirror_m(int i, int n) { return i - 1 < 0 ? (i + 1 < n ? i
 
 // The fluxes D1 (along y), D2 (along x) and, in 3D, D3 (along z) at a point
 // from u there (uc) and at its mirrored neighbours (ROF_TV_core.c:102-266).
+// eps is the floor added in double under the square root: 1e-8 for ROF-TV,
+// 1e-12 for the ROF part of LLT-ROF (LLT_ROF_core.c:22,198-363).
 template <int DIM, typename T>
-__device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T ukm, T& d1, T& d2, T& d3) {
+__device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T ukm, T& d1, T& d2, T& d3,
+                                     double eps = 1.0e-8) {
     const T fi = uip - uc, bi = uc - uim, fj = ujp - uc, bj = uc - ujm;
     const T mi = Mixed<T>::minmod_sq(fi, bi), mj = Mixed<T>::minmod_sq(fj, bj);
     if (DIM == 3) {
         const T fk = ukp - uc, bk = uc - ukm;
         const T mk = Mixed<T>::minmod_sq(fk, bk);
         // ROF_TV_core.c:131-137 (double sqrt), :194-200, :257-263
-        d1 = fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk);
-        d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi);
+        d1 = fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk, eps);
+        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk, eps);
+        d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi, eps);
     } else {
         // ROF_TV_core.c:157-161, :220-224
-        d1 = fj / Mixed<T>::rof_sqrt_f(fj * fj + mi);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj);
+        d1 = fj / Mixed<T>::rof_sqrt_f(fj * fj + mi, eps);
+        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj, eps);
         d3 = 0;
     }
 }

@@ -29,6 +29,12 @@ int sb_solve(const T* d_f, T* d_out, T mu, int iter, T epsil, int method, int nx
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws = nullptr);
+int llt_solve(const float* d_f, float* d_out, float lambda_rof, float lambda_llt, int iter, float tau, float epsil,
+              int nx, int ny, int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);
+int ndf_solve(const float* d_f, float* d_out, float lambda, float sigma, int iter, float tau, int penalty, float epsil,
+              int nx, int ny, int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);
+int d4_solve(const float* d_f, float* d_out, float lambda, float sigma, int iter, float tau, float epsil, int nx,
+             int ny, int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);
 template <typename T>
 int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsil, T eta, int method, int nonneg,
               int nx, int ny, int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re);

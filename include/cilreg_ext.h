@@ -105,6 +105,16 @@ CILREG_API int cilreg_sb_dev(const float *d_input, float *d_output, float *infov
 CILREG_API int cilreg_tgv_dev(const float *d_input, float *d_output, float *infovector, float lambda, float alpha1,
                    float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ,
                    const cilreg_opts *opts, void *stream);
+/* The explicit-diffusion family on device buffers (FP32). */
+CILREG_API int cilreg_llt_dev(const float *d_input, float *d_output, float *infovector, float lambdaROF,
+                   float lambdaLLT, int iter, float tau, float epsil, int dimX, int dimY, int dimZ,
+                   const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_ndf_dev(const float *d_input, float *d_output, float *infovector, float lambdaPar, float sigmaPar,
+                   int iter, float tau, int penaltytype, float epsil, int dimX, int dimY, int dimZ,
+                   const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_d4_dev(const float *d_input, float *d_output, float *infovector, float lambdaPar, float sigmaPar,
+                  int iter, float tau, float epsil, int dimX, int dimY, int dimZ, const cilreg_opts *opts,
+                  void *stream);
 /* FGP-dTV on device buffers (d_ref: the reference image, same dimensions);
  * FP32 storage only (dual_precision must be 0). */
 CILREG_API int cilreg_dtv_dev(const float *d_input, const float *d_ref, float *d_output, float *infovector,

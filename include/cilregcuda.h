@@ -71,6 +71,27 @@ CILREG_API int dTV_FGP_GPU_main(float *Input, float *InputRef, float *Output, fl
                      int iter, float epsil, float eta, int methodTV, int nonneg, int gpu_device, int dimX, int dimY,
                      int dimZ);
 
+/* LLT-ROF: ROF-TV plus the Lysaker-Lundervold-Tai second-order term,
+ * explicit scheme.  Replaces src/Core/regularisers_GPU/LLT_ROF_GPU_core.h:6.
+ * 2D when Z == 1 (Z == 0 is read as 1), 3D otherwise. */
+CILREG_API int LLT_ROF_GPU_main(float *Input, float *Output, float *infovector, float lambdaROF, float lambdaLLT,
+                     int iterationsNumb, float tau, float epsil, int gpu_device, int N, int M, int Z);
+
+/* NDF: linear (sigmaPar == 0) or nonlinear diffusion, penaltytype 1 Huber,
+ * 2 Perona-Malik, 3 Tukey biweight, 4 threshold-constrained linear,
+ * 5 threshold-constrained Huber.  Replaces
+ * src/Core/regularisers_GPU/NonlDiff_GPU_core.h:6.  Each step is the
+ * synchronous explicit update (every neighbour from the previous iterate);
+ * an unknown penalty type returns -1. */
+CILREG_API int NonlDiff_GPU_main(float *Input, float *Output, float *infovector, float lambdaPar, float sigmaPar,
+                      int iterationsNumb, float tau, int penaltytype, float epsil, int gpu_device, int N, int M,
+                      int Z);
+
+/* Diff4th: anisotropic fourth-order diffusion, explicit scheme.  Replaces
+ * src/Core/regularisers_GPU/Diffus_4thO_GPU_core.h:6. */
+CILREG_API int Diffus4th_GPU_main(float *Input, float *Output, float *infovector, float lambdaPar, float sigmaPar,
+                       int iterationsNumb, float tau, float epsil, int gpu_device, int N, int M, int Z);
+
 #ifdef __cplusplus
 }
 #endif

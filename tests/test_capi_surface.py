@@ -20,7 +20,8 @@ from conftest import ROOT
 
 LIB = os.path.join(ROOT, "ccpi-regularisation-toolkit_amd", "ccpi", "filters", "libcilregcuda.so")
 REFERENCE_ENTRY_POINTS = ["TV_ROF_GPU_main", "TV_FGP_GPU_main", "TV_PD_GPU_main", "TV_SB_GPU_main", "TGV_GPU_main",
-                          "PDTV_GPU_main", "SB_TV_GPU_main", "dTV_FGP_GPU_main"]
+                          "PDTV_GPU_main", "SB_TV_GPU_main", "dTV_FGP_GPU_main", "LLT_ROF_GPU_main",
+                          "NonlDiff_GPU_main", "Diffus4th_GPU_main"]
 
 
 def declared_symbols():
@@ -47,7 +48,7 @@ def test_library_exports_every_declared_symbol():
 def test_per_method_headers_match_reference_names():
     # the reference's C users include these names (regularisers_GPU/*.h)
     for h in ["TV_ROF_GPU_core.h", "TV_FGP_GPU_core.h", "TV_PD_GPU_core.h", "TV_SB_GPU_core.h", "TGV_GPU_core.h",
-              "dTV_FGP_GPU_core.h"]:
+              "dTV_FGP_GPU_core.h", "LLT_ROF_GPU_core.h", "NonlDiff_GPU_core.h", "Diffus_4thO_GPU_core.h"]:
         assert os.path.exists(os.path.join(ROOT, "include", h)), h
 
 
@@ -66,6 +67,15 @@ def test_python_surface_signatures():
         "inputData", "lambdaPar", "iterationsNumb", "epsil", "methodTV", "gpu_device"]
     assert list(inspect.signature(TV.TGV_GPU).parameters)[:8] == [
         "inputData", "lambdaPar", "alpha1", "alpha0", "iterationsNumb", "L2", "epsil", "gpu_device"]
+    # TV.py:650, diffusion.py:89-90, :133
+    assert list(inspect.signature(TV.LLT_ROF_GPU).parameters) == [
+        "inputData", "lambdaROF", "lambdaLLT", "iterationsNumb", "tau", "epsil", "gpu_device", "out", "infovector"]
+    from ccpi.filters import diffusion
+    assert list(inspect.signature(diffusion.NDF_GPU).parameters) == [
+        "inputData", "lambdaPar", "sigmaPar", "iterationsNumb", "tau", "penaltytype", "epsil", "gpu_device", "out",
+        "infovector"]
+    assert list(inspect.signature(diffusion.Diffus4th_GPU).parameters) == [
+        "inputData", "lambdaPar", "sigmaPar", "iterationsNumb", "tau", "epsil", "gpu_device", "out", "infovector"]
     # TV.py:735-736
     assert list(inspect.signature(TV.dTV_FGP_GPU).parameters) == [
         "inputData", "inputRef", "lambdaPar", "iterationsNumb", "epsil", "eta", "methodTV", "nonneg", "gpu_device",
