@@ -220,7 +220,7 @@ def dtv_section(D, torch, x):
             "kernel_frac_of_8TBps": 52 * n / (m * 1e-3) / 1e9 / HBM_PEAK_GBS if m else None}
 
 
-DIFFUSION = {   # the reference tests' parameters (test_2d_cpu_vs_gpu.py:370-378, :431-440, :496-504)
+DIFFUSION = {   # the reference tests' parameters (test_2d_cpu_vs_gpu.py:370-377, :431-438, :496-502)
     "llt": ((0.01, 0.0085), {"tau": 1e-4}, "llt_iter_kernel<3,8,uint32>"),
     "ndf": ((0.02, 0.017), {"tau": 0.01, "penalty": 1}, "ndf_iter_kernel<3,8,uint32>"),
     "d4": ((0.8, 0.02), {"tau": 1e-4}, "d4_iter_kernel<3,8,uint32>"),

@@ -131,7 +131,7 @@ def cases():
     cs.append(("llt", big, {"lambda_rof": 0.02, "lambda_llt": 0.01, "iters": 40, "tau": 0.005, "tol": 0.0}))
     cs.append(("ndf", big, {"lambda": 0.02, "sigma": 0.05, "iters": 40, "tau": 0.01, "penalty": 1, "tol": 0.0}))
     cs.append(("d4", big, {"lambda": 0.8, "sigma": 0.02, "iters": 20, "tau": 1e-4, "tol": 0.0}))
-    # the reference's 2D CPU-vs-GPU test parameters (test_2d_cpu_vs_gpu.py:370-378, :431-440, :496-504)
+    # the reference's 2D CPU-vs-GPU test parameters (test_2d_cpu_vs_gpu.py:370-377, :431-438, :496-502)
     cs.append(("llt", lena, {"lambda_rof": 0.01, "lambda_llt": 0.0085, "iters": 1000, "tau": 1e-4, "tol": 0.0}))
     cs.append(("ndf", lena, {"lambda": 0.02, "sigma": 0.017, "iters": 1500, "tau": 0.01, "penalty": 1, "tol": 0.0}))
     cs.append(("d4", lena, {"lambda": 0.8, "sigma": 0.02, "iters": 1000, "tau": 1e-4, "tol": 0.0}))
