@@ -28,6 +28,7 @@
 // plane (double-buffered across z), then updates P on plane z.  R'_3 of the
 // plane below and B of the own point stay in registers.
 #include "solvers.hpp"
+#include "fgp_kernels.hpp"
 
 namespace cilreg {
 
@@ -270,6 +271,325 @@ __global__ __launch_bounds__(TX * TY) void dtv_iter_kernel(DtvArgs<T> a) {
     dtv_body<DIM, TY, T, ISO, I>(a, tile_of_block(a.g, TY));
 }
 
+// ---------------------------------------------------------------------------
+// Two iterations per pass (3D)
+// ---------------------------------------------------------------------------
+// The FGP two-iteration pipeline (fgp2_body: ring region 67 x (TY+3), slot 0
+// the own tile point, slot 1 a ring point; stages S1 D_k(zi), S2 P_{k+1} and
+// R'_{k+1}(zi-1), S3 D_{k+1}(zi-1), S4 P_{k+2}(zi-2)) with dTV's projections.
+// R'_k needs three loads of P, P_prev and B per component, so S1 stages it in
+// LDS (S1a) and D_k reads the -x / -y neighbours from there (S1b); only the
+// region's first column and row evaluate their outside neighbour from memory.
+// R'_k and R'_{k+1} and both gradients are projected onto the complement of
+// B at their point, and B of plane zi-1 (zi-2 for S4) rides along the z
+// pipeline.  One pass reads f, B, P_k, P_{k-1} once and writes P_{k+1},
+// P_{k+2}: 64 B per voxel per two iterations instead of 2 x 52.  Every value
+// is computed by the one-iteration kernel's expressions in its order, so the
+// results are bit-identical.
+
+template <typename T>
+struct Dtv2Args {
+    const T* f;
+    const T* B[3];
+    const T* Pm[3];     // P_{k-1}
+    const T* P[3];      // P_k
+    T* Pn[3];           // P_{k+1}
+    T* Pnn[3];          // P_{k+2}
+    T* D0;              // D_k (written when write_d0)
+    T* D1;              // D_{k+1} (written when write_d1; the buffer of D_{k-1})
+    const T* Dprev;     // D_{k-1} (read when check0)
+    T lambda, step, mult0, mult1;
+    int write_d0, write_d1, check0, check1, nonneg;
+    double* partials0;
+    double* partials1;
+    const Ctrl* ctrl;
+    Geom g;
+};
+
+// v <- v - <v, B> B, the inner product summed in the reference's order
+template <typename T>
+__device__ inline void dtv_orth3(T& v1, T& v2, T& v3, T b1, T b2, T b3) {
+    T ip = v1 * b1 + v2 * b2;
+    ip = ip + v3 * b3;
+    v1 = v1 - ip * b1;
+    v2 = v2 - ip * b2;
+    v3 = v3 - ip * b3;
+}
+
+template <int TY, typename T, bool ISO, typename I>
+__device__ inline void dtv2_body(const Dtv2Args<T>& a, const TileCoord& tc) {
+    using Sh = Fgp2Shape<TY>;
+    constexpr int BW = Sh::BW, BH = Sh::BH, NR = Sh::NR, NT = Sh::NT;
+    const Geom& g = a.g;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
+
+    // slot 0: own point of the tile; slot 1: a ring point (rotated over the waves)
+    int bx[2], by[2];
+    bx[0] = lx + 1;
+    by[0] = ly + 1;
+    const int hbase = (tc.lin & 1) ? NT - NR : 0;
+    const int h = tid - hbase;
+    const bool ring = h >= 0 && h < NR;
+    {
+        int rx = 0, ry = 0;
+        if (h < 3 * BW) {
+            const int row = h / BW;
+            rx = h - row * BW;
+            ry = row == 0 ? 0 : TY + row;          // rows y0-1, y0+TY, y0+TY+1
+        } else {
+            const int k = h - 3 * BW, col = k / TY;
+            rx = col == 0 ? 0 : TX + col;          // columns x0-1, x0+TX, x0+TX+1
+            ry = k - col * TY + 1;
+        }
+        bx[1] = ring ? rx : 0;
+        by[1] = ring ? ry : 0;
+    }
+    const int nslot = ring ? 2 : 1;
+    int X[2], Y[2], cx[2], cy[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        X[s] = tc.x0 + bx[s] - 1;
+        Y[s] = tc.y0 + by[s] - 1;
+        const bool in = X[s] >= 0 && Y[s] >= 0 && X[s] < g.nx && Y[s] < g.ny;
+        cx[s] = in ? X[s] : 0;    // clamped stand-in for loads
+        cy[s] = in ? Y[s] : 0;
+    }
+    const bool own = X[0] < g.nx && Y[0] < g.ny;
+
+    __shared__ T Dk[2][BH + 1][BW + 1];    // D_k, planes by parity
+    __shared__ T Dk1[2][BH + 1][BW + 1];   // D_{k+1}
+    __shared__ T Rk1[BH + 1][BW + 1];      // R'_k,1 and ,2 of plane zi
+    __shared__ T Rk2[BH + 1][BW + 1];
+    __shared__ T R1s[BH + 1][BW + 1];      // R'_{k+1},1 and ,2 of the current plane
+    __shared__ T R2s[BH + 1][BW + 1];
+    __shared__ double red[8];
+
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const T lam = a.lambda, m0 = a.mult0, m1 = a.mult1, step = a.step;
+    const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;
+
+    // R'_k at element i, with P_k and B there
+    auto rk_at = [&](I i, T* r, T* p, T* b) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T pv = ldx<T>(a.P[c], i), qv = ldx<T>(a.Pm[c], i);
+            p[c] = pv;
+            r[c] = pv + m0 * (pv - qv);
+            b[c] = ldx<T>(a.B[c], i);
+        }
+        dtv_orth3(r[0], r[1], r[2], b[0], b[1], b[2]);
+    };
+
+    // per-slot pipeline registers
+    T rk[2][3] = {};    // R'_k(zi - 1)
+    T pk[2][3] = {};    // P_k(zi - 1)
+    T bk[2][3] = {};    // B(zi - 1)
+    T rq[2][3] = {};    // R'_{k+1}(zi - 2)
+    T bq[3] = {};       // B(zi - 2) of the own point
+    T fz[2] = {};       // f(zi - 1)
+
+    if (z0 >= 2) {   // R'_k,3 two planes below the chunk start
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (s < nslot) {
+                T r[3], p[3], b[3];
+                rk_at((I)(z0 - 2) * sxy + (I)cy[s] * nx + (I)cx[s], r, p, b);
+                rk[s][2] = r[2];
+            }
+        }
+    }
+    double sdd0 = 0.0, suu0 = 0.0, sdd1 = 0.0, suu1 = 0.0;
+    const int zstart = z0 >= 1 ? z0 - 1 : 0;
+    const int zend = z1 + 1;
+    for (int zi = zstart; zi <= zend; ++zi) {
+        // ---- S1a: R'_k(zi) on the region, into LDS
+        T rn[2][3] = {}, pn[2][3] = {}, bn[2][3] = {}, fn[2] = {};
+        const bool s1 = zi < nz;
+        if (s1) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    rk_at(i, rn[s], pn[s], bn[s]);
+                    fn[s] = ldx<T>(a.f, i);
+                    Rk1[by[s]][bx[s]] = rn[s][0];
+                    Rk2[by[s]][bx[s]] = rn[s][1];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S1b: D_k(zi); the -x / -y neighbours of the region's first
+        // column / row lie outside it and are evaluated from memory
+        if (s1) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    T b1r, b2r;
+                    if (bx[s] > 0) {
+                        b1r = Rk1[by[s]][bx[s] - 1];
+                    } else {
+                        T t[3], tp[3], tb[3];
+                        rk_at(cx[s] > 0 ? i - 1 : i, t, tp, tb);
+                        b1r = t[0];
+                    }
+                    if (by[s] > 0) {
+                        b2r = Rk2[by[s] - 1][bx[s]];
+                    } else {
+                        T t[3], tp[3], tb[3];
+                        rk_at(cy[s] > 0 ? i - nx : i, t, tp, tb);
+                        b2r = t[1];
+                    }
+                    const T b1 = X[s] > 0 ? b1r : (T)0;
+                    const T b2 = Y[s] > 0 ? b2r : (T)0;
+                    const T b3 = zi > 0 ? rk[s][2] : (T)0;
+                    T d = fn[s] - lam * (rn[s][0] + rn[s][1] + rn[s][2] - b1 - b2 - b3);
+                    if (a.nonneg && d < (T)0) d = (T)0;
+                    Dk[zi & 1][by[s]][bx[s]] = d;
+                    if (s == 0 && own && zi >= z0 && zi < z1) {
+                        const I io = (I)zi * sxy + (I)Y[0] * nx + (I)X[0];
+                        if (a.write_d0) stnt<T>(a.D0, io, d);
+                        if (a.check0) {
+                            const T df = d - ldx<T>(a.Dprev, io);
+                            sdd0 += (double)df * (double)df;
+                            suu0 += (double)d * (double)d;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S2: P_{k+1}(zc), R'_{k+1}(zc)
+        const int zc = zi - 1;
+        T rqn[2][3] = {};
+        const bool s2 = zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 && zc < nz && zc >= 0;
+        if (s2) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const int b = zc & 1;
+                    const T d = Dk[b][by[s]][bx[s]];
+                    T g1 = (X[s] == g.nx - 1) ? (T)0 : d - Dk[b][by[s]][bx[s] + 1];
+                    T g2 = (Y[s] == g.ny - 1) ? (T)0 : d - Dk[b][by[s] + 1][bx[s]];
+                    T g3 = (zc == nz - 1) ? (T)0 : d - Dk[b ^ 1][by[s]][bx[s]];
+                    dtv_orth3(g1, g2, g3, bk[s][0], bk[s][1], bk[s][2]);
+                    T p1 = rk[s][0] + step * g1;
+                    T p2 = rk[s][1] + step * g2;
+                    T p3 = rk[s][2] + step * g3;
+                    fgp_project<3, T, ISO>(p1, p2, p3);
+                    if (s == 0 && own && zc >= z0 && zc < z1) {
+                        const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
+                        stnt<T>(a.Pn[0], io, p1);
+                        stnt<T>(a.Pn[1], io, p2);
+                        stnt<T>(a.Pn[2], io, p3);
+                    }
+                    rqn[s][0] = p1 + m1 * (p1 - pk[s][0]);
+                    rqn[s][1] = p2 + m1 * (p2 - pk[s][1]);
+                    rqn[s][2] = p3 + m1 * (p3 - pk[s][2]);
+                    dtv_orth3(rqn[s][0], rqn[s][1], rqn[s][2], bk[s][0], bk[s][1], bk[s][2]);
+                    R1s[by[s]][bx[s]] = rqn[s][0];
+                    R2s[by[s]][bx[s]] = rqn[s][1];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S3: D_{k+1}(zc)
+        const bool s3 = zc >= z0 && zc <= z1 && zc < nz;
+        if (s3) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const T b1r = R1s[by[s]][bx[s] > 0 ? bx[s] - 1 : 0];
+                    const T b2r = R2s[by[s] > 0 ? by[s] - 1 : 0][bx[s]];
+                    const T b1 = X[s] > 0 ? b1r : (T)0;
+                    const T b2 = Y[s] > 0 ? b2r : (T)0;
+                    const T b3 = zc > 0 ? rq[s][2] : (T)0;
+                    T d = fz[s] - lam * (rqn[s][0] + rqn[s][1] + rqn[s][2] - b1 - b2 - b3);
+                    if (a.nonneg && d < (T)0) d = (T)0;
+                    Dk1[zc & 1][by[s]][bx[s]] = d;
+                    if (s == 0 && own && zc < z1) {
+                        const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
+                        if (a.write_d1) stnt<T>(a.D1, io, d);
+                        if (a.check1) {
+                            const T df = d - Dk[zc & 1][by[0]][bx[0]];
+                            sdd1 += (double)df * (double)df;
+                            suu1 += (double)d * (double)d;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S4: P_{k+2}(zq) on the tile
+        const int zq = zi - 2;
+        if (zq >= z0 && zq < z1 && own) {
+            const int b = zq & 1;
+            const T d = Dk1[b][by[0]][bx[0]];
+            T g1 = (X[0] == g.nx - 1) ? (T)0 : d - Dk1[b][by[0]][bx[0] + 1];
+            T g2 = (Y[0] == g.ny - 1) ? (T)0 : d - Dk1[b][by[0] + 1][bx[0]];
+            T g3 = (zq == nz - 1) ? (T)0 : d - Dk1[b ^ 1][by[0]][bx[0]];
+            dtv_orth3(g1, g2, g3, bq[0], bq[1], bq[2]);
+            T p1 = rq[0][0] + step * g1;
+            T p2 = rq[0][1] + step * g2;
+            T p3 = rq[0][2] + step * g3;
+            fgp_project<3, T, ISO>(p1, p2, p3);
+            const I io = (I)zq * sxy + (I)Y[0] * nx + (I)X[0];
+            stnt<T>(a.Pnn[0], io, p1);
+            stnt<T>(a.Pnn[1], io, p2);
+            stnt<T>(a.Pnn[2], io, p3);
+        }
+        // ---- shift the pipeline
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (s2) bq[c] = bk[0][c];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (s1) { rk[s][c] = rn[s][c]; pk[s][c] = pn[s][c]; bk[s][c] = bn[s][c]; }
+                if (s2) rq[s][c] = rqn[s][c];
+            }
+            if (s1) fz[s] = fn[s];
+        }
+    }
+    if (a.check0) {
+        block_sum2(sdd0, suu0, red);
+        if (tid == 0) {
+            a.partials0[2 * tc.lin] = sdd0;
+            a.partials0[2 * tc.lin + 1] = suu0;
+        }
+        __syncthreads();
+    }
+    if (a.check1) {
+        block_sum2(sdd1, suu1, red);
+        if (tid == 0) {
+            a.partials1[2 * tc.lin] = sdd1;
+            a.partials1[2 * tc.lin + 1] = suu1;
+        }
+    }
+}
+
+template <int TY, typename T, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void dtv2_iter_kernel(Dtv2Args<T> a) {
+    if (solve_stopped(a.ctrl)) return;
+    dtv2_body<TY, T, ISO, I>(a, tile_of_block(a.g, TY));
+}
+
+template <int TY, typename T>
+static const void* dtv2_kernel_ptr(bool iso, bool small) {
+    if (small)
+        return iso ? (const void*)dtv2_iter_kernel<TY, T, true, uint32_t> : (const void*)dtv2_iter_kernel<TY, T, false, uint32_t>;
+    return iso ? (const void*)dtv2_iter_kernel<TY, T, true, uint64_t> : (const void*)dtv2_iter_kernel<TY, T, false, uint64_t>;
+}
+
+template <int TY, typename T>
+static void launch_dtv2(const Dtv2Args<T>& a, bool iso, bool small, hipStream_t s) {
+    dim3 block(TX, TY), grid(a.g.nblocks);
+    Dtv2Args<T> arg = a;
+    void* args[] = {&arg};
+    (void)hipLaunchKernel(dtv2_kernel_ptr<TY, T>(iso, small), grid, block, args, 0, s);
+}
+
 template <int DIM, int TY, typename T, typename I>
 static const void* dtv_kernel_ptr(bool iso) {
     return iso ? (const void*)dtv_iter_kernel<DIM, TY, T, true, I> : (const void*)dtv_iter_kernel<DIM, TY, T, false, I>;
@@ -302,9 +622,14 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T));
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
+    const int width = (dim == 3 && opts.depth != 1 && iter > 1) ? 2 : 1;
     Geom g;
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
+    else if (width == 2)
+        // the pass evaluates about three planes below and above its chunk
+        g = ty == 8 ? plan_geom(dtv2_kernel_ptr<8, T>(iso, small), TX * 8, nx, ny, nz, 8, opts.zchunk, 3.0)
+                    : plan_geom(dtv2_kernel_ptr<4, T>(iso, small), TX * 4, nx, ny, nz, 4, opts.zchunk, 3.0);
     else if (ty == 8)
         g = plan_geom(small ? dtv_kernel_ptr<3, 8, T, uint32_t>(iso) : dtv_kernel_ptr<3, 8, T, uint64_t>(iso), TX * 8,
                       nx, ny, nz, 8, opts.zchunk);
@@ -312,20 +637,23 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
         g = plan_geom(small ? dtv_kernel_ptr<3, 4, T, uint32_t>(iso) : dtv_kernel_ptr<3, 4, T, uint64_t>(iso), TX * 4,
                       nx, ny, nz, 4, opts.zchunk);
 
+    // P_j lives in buffer j mod nbuf (4 with two-iteration passes, which write
+    // P_{k+1} and P_{k+2} while reading P_k and P_{k-1})
+    const int nbuf = width == 2 ? 4 : 3;
     Arena ar;
-    size_t oP[3][3], oB[3], oD[2], oCtrl, oPart;
-    for (int b = 0; b < 3; ++b)
+    size_t oP[4][3], oB[3], oD[2], oCtrl, oPart;
+    for (int b = 0; b < nbuf; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(T));
     for (int c = 0; c < dim; ++c) oB[c] = ar.reserve(n * sizeof(T));
     for (int b = 0; b < 2; ++b) oD[b] = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
-    oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
+    oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
     for (int c = 0; c < dim; ++c) {   // P_0 = P_{-1} = 0
         launch_zero(ar.at(oP[0][c]), n * sizeof(T), s);
-        launch_zero(ar.at(oP[2][c]), n * sizeof(T), s);
+        launch_zero(ar.at(oP[nbuf - 1][c]), n * sizeof(T), s);
     }
-    auto Pbuf = [&](int j, int c) -> T* { return (T*)ar.at(oP[((j % 3) + 3) % 3][c < dim ? c : 0]); };
+    auto Pbuf = [&](int j, int c) -> T* { return (T*)ar.at(oP[((j % nbuf) + nbuf) % nbuf][c < dim ? c : 0]); };
     T* B[3];
     for (int c = 0; c < 3; ++c) B[c] = (T*)ar.at(oB[c < dim ? c : 0]);
     {
@@ -343,6 +671,7 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
     if (use_tol) launch_copy(Dbuf[1], d_out, n * sizeof(T), s);   // D_{-1} = caller's Output
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
     double* partials = (double*)ar.at(oPart);
+    double* partials1 = partials + 2 * (size_t)g.nblocks;
     const int ce = opts.check_every;
     auto needs_d = [&](int k) { return k == iter - 1 || (use_tol && (k % ce == 0 || (k + 1) % ce == 0)); };
     const T stepc = (T)1 / ((dim == 3 ? (T)26 : (T)8) * lambda);
@@ -359,7 +688,37 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
             tk = tkp1;
         }
     }
-    auto step = [&](int k, bool check) {
+    auto step = [&](int k, int nk, bool check, bool check1) {
+        if (nk == 2) {
+            Dtv2Args<T> a;
+            a.f = d_f;
+            for (int c = 0; c < 3; ++c) {
+                a.B[c] = B[c];
+                a.Pm[c] = Pbuf(k - 1, c);
+                a.P[c] = Pbuf(k, c);
+                a.Pn[c] = Pbuf(k + 1, c);
+                a.Pnn[c] = Pbuf(k + 2, c);
+            }
+            a.D0 = Dbuf[k & 1];
+            a.D1 = Dbuf[(k + 1) & 1];
+            a.Dprev = Dbuf[(k + 1) & 1];
+            a.lambda = lambda;
+            a.step = stepc;
+            a.mult0 = mult[k];
+            a.mult1 = mult[k + 1];
+            a.write_d0 = needs_d(k) || check;
+            a.write_d1 = needs_d(k + 1) || check1;
+            a.check0 = check;
+            a.check1 = check1;
+            a.nonneg = nonneg != 0;
+            a.partials0 = partials;
+            a.partials1 = partials1;
+            a.ctrl = ctrl;
+            a.g = g;
+            if (ty == 8) launch_dtv2<8>(a, iso, small, s);
+            else launch_dtv2<4>(a, iso, small, s);
+            return;
+        }
         DtvArgs<T> a;
         a.f = d_f;
         for (int c = 0; c < 3; ++c) {
@@ -387,7 +746,8 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
         }
     };
     int done = 0;
-    if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
+    if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials, partials1, g.nblocks, s, step, &done, re) != 0)
+        return -1;
     *iters_done = done;
     const int last = (done < iter) ? done : iter - 1;
     CILREG_CK(hipMemcpyAsync(d_out, Dbuf[last & 1], n * sizeof(T), hipMemcpyDeviceToDevice, s));

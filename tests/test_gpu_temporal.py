@@ -1,4 +1,5 @@
-"""Two-iteration FGP passes (temporal blocking) against one-iteration passes.
+"""Two-iteration FGP, PD, TGV and FGP-dTV passes (temporal blocking) against
+one-iteration passes.
 
 The 3D FGP solver fuses iterations k and k+1 into one pass by default; the
 pass evaluates iteration k on a ring around its tile and every plane through
@@ -171,3 +172,55 @@ def test_pd_depth2_wide_index():
     x = _vol((12, 30, 66), 34)
     a, b = _pd_pair(x, lam=0.03, iters=9, wide_index=True)
     _same(a, b, "PD wide index")
+
+
+# ---------------------------------------------------------------------------
+# FGP-dTV: the FGP pipeline with dTV's projections onto the complement of the
+# reference gradient B; depth 2 == depth 1 bit for bit.
+# ---------------------------------------------------------------------------
+def _dtv_pair(x, r, **kw):
+    from ccpi.filters import device as D
+    a = D.dtv(x, r, depth=1, **kw)
+    b = D.dtv(x, r, depth=2, **kw)
+    return a, b
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("zchunk", [0, 1, 3])
+def test_dtv_depth2_matches_depth1_tilings(shape, zchunk):
+    x = _vol(shape, 21)
+    r = _vol(shape, 22)
+    for tile_y in (4, 8):
+        for iters in (1, 2, 7, 10):
+            a, b = _dtv_pair(x, r, lam=0.03, iters=iters, eta=0.05, zchunk=zchunk, tile_y=tile_y, nonneg=1)
+            _same(a, b, f"dtv {shape} zchunk={zchunk} tile_y={tile_y} iters={iters}")
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["iso", "aniso"])
+def test_dtv_depth2_matches_depth1_methods(method):
+    x = _vol((24, 40, 72), 23)
+    r = _vol((24, 40, 72), 24)
+    for nonneg in (0, 1):
+        for eta in (0.01, 0.3):
+            a, b = _dtv_pair(x, r, lam=0.02, iters=25, eta=eta, method=method, nonneg=nonneg)
+            _same(a, b, f"dtv method={method} nonneg={nonneg} eta={eta}")
+
+
+@pytest.mark.parametrize("check_every", [1, 2, 5])
+def test_dtv_depth2_matches_depth1_tolerance_stops(check_every):
+    x = _vol((16, 48, 80), 25)
+    r = _vol((16, 48, 80), 26)
+    stopped = False
+    for tol in (1e-2, 3e-3, 1e-4):
+        for zchunk in (0, 4):
+            a, b = _dtv_pair(x, r, lam=0.03, iters=120, tol=tol, eta=0.05, check_every=check_every, zchunk=zchunk)
+            _same(a, b, f"dtv tol={tol} check_every={check_every} zchunk={zchunk}")
+            stopped |= int(a[1][0]) < 120
+    assert stopped
+
+
+def test_dtv_depth2_wide_index():
+    x = _vol((12, 30, 66), 27)
+    r = _vol((12, 30, 66), 28)
+    a, b = _dtv_pair(x, r, lam=0.03, iters=9, eta=0.05, wide_index=True)
+    _same(a, b, "dtv wide index")
