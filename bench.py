@@ -29,7 +29,7 @@ Also reported:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu] [--no-batch]
                        [--no-serve] [--serve-window SECONDS]
        python bench.py --profile-pass --steps K --warmup 0   (the headline solves only)
-       python bench.py --workload {tgv3d,pd3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
+       python bench.py --workload {tgv3d,pd3d,sb3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
        profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
        known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -175,6 +175,20 @@ def secondary(D, torch):
         "voxel_iters_per_s": n * info[0] / dt, "iterations": int(info[0]), "seconds": round(dt, 4),
         "kernel_ms_mean": m, "alg_bytes_per_voxel_iter": 36,
         "kernel_alg_GBps": 36 * n / (m * 1e-3) / 1e9 if m else None}
+    # SB 3D (two kernels per iteration)
+    log("bench: SB-TV 3D 256^3")
+    D.sb(x, 0.02, 10, 0.0, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, info = D.sb(x, 0.02, 150, 1e-6, 0, check_every=1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    D.sb(x, 0.02, 30, 0.0, 0, profile=True)
+    m, _, _ = D.last_profile()
+    out["sb_tv_3d_256_fp32"] = {
+        "voxel_iters_per_s": n * info[0] / dt, "iterations": int(info[0]), "seconds": round(dt, 4),
+        "kernel_ms_mean": m, "alg_bytes_per_voxel_iter": 96,
+        "kernel_alg_GBps": 96 * n / (m * 1e-3) / 1e9 if m else None}
     # FGP-dTV 3D, guided by the clean phantom with 1% noise
     log("bench: FGP-dTV 3D 256^3")
     out["fgp_dtv_3d_256_fp32"] = dtv_section(D, torch, x)
@@ -462,6 +476,12 @@ def single_workload(name, steps, D, torch):
         D.pd(x, 0.02, steps, 0.0, 12.0, 0, 1, profile=True)
         m, _, c = D.last_profile()
         return {"workload": "PD-TV 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c}
+    if name == "sb3d":
+        x = torch.from_numpy(fgp_workload()).cuda()
+        D.sb(x, 0.02, steps, 0.0, 0, profile=True)
+        m, _, c = D.last_profile()
+        return {"workload": "SB-TV 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c,
+                "kernel_alg_GBps": 96 * x.numel() / (m * 1e-3) / 1e9 if m else None}
     if name in ("llt3d", "ndf3d", "d43d"):
         x = torch.from_numpy(fgp_workload()).cuda()
         run_diffusion(D, name[:-2], x, steps, profile=True)
@@ -492,8 +512,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-window", type=float, default=0.5,
                     help="seconds of Poisson arrivals replayed by the served section")
-    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "dtv3d", "llt3d", "ndf3d",
-                                                                "d43d", "copy", "batch", "serve"])
+    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "sb3d", "dtv3d", "llt3d",
+                                                                "ndf3d", "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
