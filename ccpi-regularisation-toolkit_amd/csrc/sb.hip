@@ -1,4 +1,5 @@
-// sb.hip -- SB-TV (split Bregman), two kernels per iteration.
+// sb.hip -- SB-TV (split Bregman), one fused kernel per iteration (two
+// kernels for extents whose last tile is one column or row wide).
 //
 // Reference algorithm: src/Core/regularisers_CPU/SB_TV_core.c:38-371.  With
 // mu' = 1/mu and lam = 2 mu', per iteration:
@@ -11,7 +12,7 @@
 //   B = B + (U2(x + e_d) - U2(x)) - D      (:233-248, :353-371)
 // The residual compares U2 with U1 (:123-133).
 //
-// Kernel 1 runs the first sweep.  Kernel 2 runs the second sweep on the tile
+// Two-kernel path: kernel 1 runs the first sweep.  Kernel 2 runs the second sweep on the tile
 // plus its +x / +y halo into an LDS plane (double-buffered across z), then the
 // shrinkage and Bregman update of the tile.  HBM traffic per voxel-iteration:
 // sweep 1 reads U0, f, D(3), B(3) and writes U1 (36 B); kernel 2 reads U1, f,
@@ -187,6 +188,217 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused pass: both sweeps, the shrinkage and the Bregman update in one kernel
+// ---------------------------------------------------------------------------
+// The D/B part of a sweep value, sum_d (D_d(m-) - D_d - B_d(m-) + B_d) in the
+// reference's order, depends only on the old D and B, so both sweeps at a
+// point share it: U_new = nc ((db + sum of six U neighbours) lam + mu f).
+// The workgroup's region is the 67 x (TY+3) ring region of the FGP
+// two-iteration kernels (slot 0 the own tile point, slot 1 a ring point).
+//   S1  U1(zi) = J(U0) on the region, into a 4-plane LDS ring; db and f kept
+//   S2  U2(zi-1) = J(U1) on the tile and its +x column / +y row (from LDS)
+//   S3  shrink + Bregman on the tile at zi-2: forward differences of U2 from
+//       LDS (mirrored at the last index), write U2, D, B
+// HBM traffic per voxel-iteration: read U0, f, D(3), B(3); write U2, D(3),
+// B(3) = 60 B (the two-kernel path moves 96 B).  The mirrored neighbour of
+// the last index lies in the region unless the last tile is one column (row)
+// wide; those extents use the two-kernel path.
+
+// the D/B part of the sweep at element c with mirrored -x / -y / -z elements
+template <int DIM, typename I, typename T>
+__device__ inline T sb_db(const SbArgs<T>& a, I c, I cim, I cjm, I ckm) {
+    auto L = [](const T* p, I e) { return ldx<T>(p, e); };
+    if (DIM == 2)
+        return L(a.D[0], cim) - L(a.D[0], c) + L(a.D[1], cjm) - L(a.D[1], c) - L(a.B[0], cim) + L(a.B[0], c) -
+               L(a.B[1], cjm) + L(a.B[1], c);
+    const T dsum = L(a.D[0], cim) - L(a.D[0], c) + L(a.D[1], cjm) - L(a.D[1], c) + L(a.D[2], ckm) - L(a.D[2], c);
+    const T bsum = -L(a.B[0], cim) + L(a.B[0], c) - L(a.B[1], cjm) + L(a.B[1], c) - L(a.B[2], ckm) + L(a.B[2], c);
+    return dsum + bsum;
+}
+
+template <typename T>
+__device__ inline T sb_combine(const SbArgs<T>& a, T db, T usum, T fv) {
+    T acc = db;
+    acc += usum;
+    acc *= a.lam;
+    acc += a.mu * fv;
+    return a.nc * acc;
+}
+
+template <int DIM, int TY, typename T, bool ANISO, typename I>
+__global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
+    if (solve_stopped(a.ctrl)) return;
+    const Geom& g = a.g;
+    const TileCoord tc = tile_of_block(g, TY);
+    constexpr int BW = TX + 3, BH = TY + 3, NR = 3 * BW + 3 * TY, NT = TX * TY;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
+
+    int bx[2], by[2];
+    bx[0] = lx + 1;
+    by[0] = ly + 1;
+    const int hbase = (tc.lin & 1) ? NT - NR : 0;
+    const int h = tid - hbase;
+    const bool ring = h >= 0 && h < NR;
+    {
+        int rx = 0, ry = 0;
+        if (h < 3 * BW) {
+            const int row = h / BW;
+            rx = h - row * BW;
+            ry = row == 0 ? 0 : TY + row;          // rows y0-1, y0+TY, y0+TY+1
+        } else {
+            const int k = h - 3 * BW, col = k / TY;
+            rx = col == 0 ? 0 : TX + col;          // columns x0-1, x0+TX, x0+TX+1
+            ry = k - col * TY + 1;
+        }
+        bx[1] = ring ? rx : 0;
+        by[1] = ring ? ry : 0;
+    }
+    const int nslot = ring ? 2 : 1;
+    int X[2], Y[2], cx[2], cy[2];
+    bool in[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        X[s] = tc.x0 + bx[s] - 1;
+        Y[s] = tc.y0 + by[s] - 1;
+        in[s] = X[s] >= 0 && Y[s] >= 0 && X[s] < g.nx && Y[s] < g.ny;
+        cx[s] = in[s] ? X[s] : 0;   // clamped stand-in
+        cy[s] = in[s] ? Y[s] : 0;
+    }
+    const bool own = in[0];
+    // slot 1 also evaluates U2 on the +x column and the +y row
+    const bool u2halo = ring && in[1] &&
+                        ((by[1] == TY + 1 && bx[1] >= 1 && bx[1] <= TX) || (bx[1] == TX + 1 && by[1] >= 1 && by[1] <= TY));
+    // mirrored x / y neighbour slots (SB's rule: +1, or -1 at the last index)
+    int sxp[2], sxm[2], syp[2], sym[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        sxp[s] = sb_p(cx[s], g.nx) - tc.x0 + 1;
+        sxm[s] = sb_m(cx[s], g.nx) - tc.x0 + 1;
+        syp[s] = sb_p(cy[s], g.ny) - tc.y0 + 1;
+        sym[s] = sb_m(cy[s], g.ny) - tc.y0 + 1;
+    }
+
+    __shared__ T U1p[4][BH][BW];   // U1 planes by z mod 4
+    __shared__ T U2p[2][BH][BW];   // U2 planes by z mod 2
+    __shared__ double red[8];
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;
+
+    T dbz[2] = {}, fz[2] = {};     // db and f of plane zi - 1
+    double sdd = 0.0, suu = 0.0;
+    const int zstart = z0 >= 2 ? z0 - 2 : 0;
+    for (int zi = zstart; zi <= z1 + 1; ++zi) {
+        // ---- S1: U1(zi) on the region
+        T dbn[2] = {}, fn[2] = {};
+        const bool s1 = zi < nz;
+        if (s1) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const I zo = (I)zi * sxy, row = (I)cy[s] * nx;
+                    const I c = zo + row + (I)cx[s];
+                    const I cim = zo + row + (I)sb_m(cx[s], g.nx), cip = zo + row + (I)sb_p(cx[s], g.nx);
+                    const I cjm = zo + (I)sb_m(cy[s], g.ny) * nx + (I)cx[s];
+                    const I cjp = zo + (I)sb_p(cy[s], g.ny) * nx + (I)cx[s];
+                    I ckm = c, ckp = c;
+                    if (DIM == 3) {
+                        ckm = (I)sb_m(zi, nz) * sxy + row + (I)cx[s];
+                        ckp = (I)sb_p(zi, nz) * sxy + row + (I)cx[s];
+                    }
+                    dbn[s] = sb_db<DIM, I>(a, c, cim, cjm, ckm);
+                    fn[s] = ldx<T>(a.f, c);
+                    const T* U = a.Uin;
+                    T us = ldx<T>(U, cip) + ldx<T>(U, cim) + ldx<T>(U, cjp) + ldx<T>(U, cjm);
+                    if (DIM == 3) us = us + ldx<T>(U, ckp) + ldx<T>(U, ckm);
+                    U1p[zi & 3][by[s]][bx[s]] = sb_combine(a, dbn[s], us, fn[s]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S2: U2(zc) on the tile and its +x / +y halo
+        const int zc = zi - 1;
+        if (zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 && zc < nz) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                // only points inside the volume: their mirrored neighbours are region slots
+                if (s == 0 ? own : u2halo) {
+                    const int b = zc & 3;
+                    T us = U1p[b][by[s]][sxp[s]] + U1p[b][by[s]][sxm[s]] + U1p[b][syp[s]][bx[s]] +
+                           U1p[b][sym[s]][bx[s]];
+                    if (DIM == 3)
+                        us = us + U1p[sb_p(zc, nz) & 3][by[s]][bx[s]] + U1p[sb_m(zc, nz) & 3][by[s]][bx[s]];
+                    U2p[zc & 1][by[s]][bx[s]] = sb_combine(a, dbz[s], us, fz[s]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S3: shrinkage and Bregman update on the tile at zq
+        const int zq = zi - 2;
+        if (zq >= z0 && zq < z1 && own) {
+            const I i = (I)zq * sxy + (I)Y[0] * nx + (I)X[0];
+            const T u2 = U2p[zq & 1][by[0]][bx[0]];
+            T gr[3], v[3], dn[3], bv[3];
+            gr[0] = U2p[zq & 1][by[0]][sxp[0]] - u2;
+            gr[1] = U2p[zq & 1][syp[0]][bx[0]] - u2;
+            gr[2] = 0;
+            if (DIM == 3) gr[2] = U2p[sb_p(zq, nz) & 1][by[0]][bx[0]] - u2;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                bv[d] = ldx<T>(a.B[d], i);
+                v[d] = gr[d] + bv[d];
+            }
+            if (ANISO) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    T sh = fabs(v[d]) - a.thr;
+                    if (sh < (T)0) sh = 0;
+                    dn[d] = (v[d] != (T)0) ? (v[d] / fabs(v[d])) * sh : (T)0;
+                }
+            } else {
+                T n2 = v[0] * v[0] + v[1] * v[1];
+                if (DIM == 3) n2 = n2 + v[2] * v[2];
+                const T nrm = dsqrt(n2);
+                T sh = nrm - a.thr;
+                if (sh < (T)0) sh = 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) dn[d] = (nrm != (T)0) ? sh * (v[d] / nrm) : (T)0;
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                stnt<T>(a.Dout[d], i, dn[d]);
+                stnt<T>(a.Bout[d], i, bv[d] + (gr[d] - dn[d]));
+            }
+            stnt<T>(a.Uout, i, u2);
+            if (a.check) {
+                const T df = u2 - U1p[zq & 3][by[0]][bx[0]];
+                sdd += (double)df * (double)df;
+                suu += (double)u2 * (double)u2;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (s1) { dbz[s] = dbn[s]; fz[s] = fn[s]; }
+        }
+    }
+    if (a.check) {
+        block_sum2(sdd, suu, red);
+        if (tid == 0) {
+            a.partials[2 * tc.lin] = sdd;
+            a.partials[2 * tc.lin + 1] = suu;
+        }
+    }
+}
+
+template <int DIM, int TY, typename T>
+static const void* sb_fused_ptr(bool aniso, bool small) {
+    if (small)
+        return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint32_t>
+                     : (const void*)sb_fused_kernel<DIM, TY, T, false, uint32_t>;
+    return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint64_t>
+                 : (const void*)sb_fused_kernel<DIM, TY, T, false, uint64_t>;
+}
+
 template <typename T, typename S>
 int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int nx, int ny, int nz, const Opts& opts,
              hipStream_t s, int* iters_done, float* re) {
@@ -201,11 +413,23 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
         return 0;
     }
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T));
-    // the SB kernels are built for 64 x 4 tiles
-    const Geom g = dim == 2 ? make_geom(nx, ny, nz, 4, 1)
-                            : plan_geom(small ? (const void*)sb_update_kernel<3, 4, T, false, uint32_t>
-                                              : (const void*)sb_update_kernel<3, 4, T, false, uint64_t>,
-                                        TX * 4, nx, ny, nz, 4, opts.zchunk);
+    const bool aniso = method == 1;
+    // one fused pass per iteration, unless the last tile is one column / row
+    // wide (its mirrored neighbour lies outside the region) or depth 1 asks
+    // for one pass per sweep
+    const int fty = (dim == 3 && opts.tile_y != 4) ? 8 : 4;
+    const bool fused = opts.depth != 1 && (nx == 1 || nx % TX != 1) && (ny == 1 || ny % fty != 1);
+    Geom g;
+    if (fused)
+        g = dim == 2 ? make_geom(nx, ny, nz, 4, 1)
+                     : (fty == 8 ? plan_geom(sb_fused_ptr<3, 8, T>(aniso, small), TX * 8, nx, ny, nz, 8, opts.zchunk, 2.0)
+                                 : plan_geom(sb_fused_ptr<3, 4, T>(aniso, small), TX * 4, nx, ny, nz, 4, opts.zchunk, 2.0));
+    else
+        // the two-kernel path is built for 64 x 4 tiles
+        g = dim == 2 ? make_geom(nx, ny, nz, 4, 1)
+                     : plan_geom(small ? (const void*)sb_update_kernel<3, 4, T, false, uint32_t>
+                                       : (const void*)sb_update_kernel<3, 4, T, false, uint64_t>,
+                                 TX * 4, nx, ny, nz, 4, opts.zchunk);
     Arena ar;
     size_t oU0, oU1, oD[2][3], oB[2][3], oCtrl, oPart;
     oU0 = ar.reserve(n * sizeof(T));
@@ -248,6 +472,18 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
         a.partials = partials;
         a.ctrl = ctrl;
         a.g = g;
+        if (fused) {   // U_k -> U_{k+1}, ping-pong between the two U buffers
+            a.Uin = (const T*)ar.at((k & 1) ? oU1 : oU0);
+            a.Uout = (T*)ar.at((k & 1) ? oU0 : oU1);
+            a.check = check;
+            dim3 grid(g.nblocks), block(TX, fty);
+            SbArgs<T> arg = a;
+            void* args[] = {&arg};
+            const void* kp = dim == 3 ? (fty == 8 ? sb_fused_ptr<3, 8, T>(aniso, small) : sb_fused_ptr<3, 4, T>(aniso, small))
+                                      : sb_fused_ptr<2, 4, T>(aniso, small);
+            (void)hipLaunchKernel(kp, grid, block, args, 0, s);
+            return;
+        }
         dim3 grid(g.nblocks), block(TX, 4);
         // sweep 1: U0 -> U1
         a.Uin = (const T*)ar.at(oU0);
@@ -285,7 +521,10 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
     int done = 0;
     if (drive_iterations(iter, (float)epsil, opts, ctrl, partials, g.nblocks, s, step, &done, re) != 0) return -1;
     *iters_done = done;
-    CILREG_CK(hipMemcpyAsync(d_out, ar.at(oU0), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    // the fused path leaves U of the last executed iteration in buffer (last + 1) mod 2
+    const int last = (done < iter) ? done : iter - 1;
+    const size_t oRes = (fused && ((last + 1) & 1)) ? oU1 : oU0;
+    CILREG_CK(hipMemcpyAsync(d_out, ar.at(oRes), n * sizeof(T), hipMemcpyDeviceToDevice, s));
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
