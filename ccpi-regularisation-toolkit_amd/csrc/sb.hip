@@ -196,7 +196,9 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
 // point share it: U_new = nc ((db + sum of six U neighbours) lam + mu f).
 // The workgroup's region is the 67 x (TY+3) ring region of the FGP
 // two-iteration kernels (slot 0 the own tile point, slot 1 a ring point).
-//   S1  U1(zi) = J(U0) on the region, into a 4-plane LDS ring; db and f kept
+//   S1  U1(zi) = J(U0) on the region, into a 4-plane LDS ring; db and f kept.
+//       The plane's U0, D_1, B_1, D_2, B_2 go through LDS for the x / y
+//       neighbours, U0 above / below and D_3, B_3 below ride in registers
 //   S2  U2(zi-1) = J(U1) on the tile and its +x column / +y row (from LDS)
 //   S3  shrink + Bregman on the tile at zi-2: forward differences of U2 from
 //       LDS (mirrored at the last index), write U2, D, B
@@ -281,37 +283,88 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
 
     __shared__ T U1p[4][BH][BW];   // U1 planes by z mod 4
     __shared__ T U2p[2][BH][BW];   // U2 planes by z mod 2
+    __shared__ T U0s[BH][BW];      // U0, D_1, B_1, D_2, B_2 of plane zi (x / y neighbours of S1)
+    __shared__ T D0s[BH][BW], B0s[BH][BW], D1s[BH][BW], B1s[BH][BW];
     __shared__ double red[8];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;
 
+    // per-slot z pipeline of U0 and of D_3, B_3 below (mirrored at the ends)
+    T u0m[2] = {}, u0c[2] = {}, d2m[2] = {}, b2m[2] = {};
+    const int zstart = z0 >= 2 ? z0 - 2 : 0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s < nslot) {
+            const I col = (I)cy[s] * nx + (I)cx[s];
+            u0c[s] = ldx<T>(a.Uin, (I)zstart * sxy + col);
+            if (DIM == 3) {
+                const I cm = (I)sb_m(zstart, nz) * sxy + col;
+                u0m[s] = ldx<T>(a.Uin, cm);
+                d2m[s] = ldx<T>(a.D[2], cm);
+                b2m[s] = ldx<T>(a.B[2], cm);
+            }
+        }
+    }
     T dbz[2] = {}, fz[2] = {};     // db and f of plane zi - 1
     double sdd = 0.0, suu = 0.0;
-    const int zstart = z0 >= 2 ? z0 - 2 : 0;
     for (int zi = zstart; zi <= z1 + 1; ++zi) {
-        // ---- S1: U1(zi) on the region
-        T dbn[2] = {}, fn[2] = {};
+        // ---- S1a: own values of plane zi into LDS; U0 of the plane above
+        T dbn[2] = {}, fn[2] = {}, u0p[2] = {}, dv[2][3] = {}, bv[2][3] = {};
         const bool s1 = zi < nz;
         if (s1) {
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (s < nslot) {
+                    const I col = (I)cy[s] * nx + (I)cx[s];
+                    const I c = (I)zi * sxy + col;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        dv[s][d] = ldx<T>(a.D[d], c);
+                        bv[s][d] = ldx<T>(a.B[d], c);
+                    }
+                    fn[s] = ldx<T>(a.f, c);
+                    if (DIM == 3) u0p[s] = (zi + 1 < nz) ? ldx<T>(a.Uin, (I)(zi + 1) * sxy + col) : u0m[s];
+                    U0s[by[s]][bx[s]] = u0c[s];
+                    D0s[by[s]][bx[s]] = dv[s][0];
+                    B0s[by[s]][bx[s]] = bv[s][0];
+                    D1s[by[s]][bx[s]] = dv[s][1];
+                    B1s[by[s]][bx[s]] = bv[s][1];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S1b: U1(zi) = J(U0) on the region; x / y neighbours from LDS,
+        // or from memory where the mirrored neighbour lies outside the region
+        if (s1) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
                     const I zo = (I)zi * sxy, row = (I)cy[s] * nx;
-                    const I c = zo + row + (I)cx[s];
+                    const bool xm_in = sxm[s] >= 0 && sxm[s] < BW, xp_in = sxp[s] >= 0 && sxp[s] < BW;
+                    const bool ym_in = sym[s] >= 0 && sym[s] < BH, yp_in = syp[s] >= 0 && syp[s] < BH;
                     const I cim = zo + row + (I)sb_m(cx[s], g.nx), cip = zo + row + (I)sb_p(cx[s], g.nx);
                     const I cjm = zo + (I)sb_m(cy[s], g.ny) * nx + (I)cx[s];
                     const I cjp = zo + (I)sb_p(cy[s], g.ny) * nx + (I)cx[s];
-                    I ckm = c, ckp = c;
-                    if (DIM == 3) {
-                        ckm = (I)sb_m(zi, nz) * sxy + row + (I)cx[s];
-                        ckp = (I)sb_p(zi, nz) * sxy + row + (I)cx[s];
+                    const T d0m = xm_in ? D0s[by[s]][sxm[s]] : ldx<T>(a.D[0], cim);
+                    const T b0m = xm_in ? B0s[by[s]][sxm[s]] : ldx<T>(a.B[0], cim);
+                    const T d1m = ym_in ? D1s[sym[s]][bx[s]] : ldx<T>(a.D[1], cjm);
+                    const T b1m = ym_in ? B1s[sym[s]][bx[s]] : ldx<T>(a.B[1], cjm);
+                    T db;
+                    if (DIM == 2) {
+                        db = d0m - dv[s][0] + d1m - dv[s][1] - b0m + bv[s][0] - b1m + bv[s][1];
+                    } else {
+                        const T dsum = d0m - dv[s][0] + d1m - dv[s][1] + d2m[s] - dv[s][2];
+                        const T bsum = -b0m + bv[s][0] - b1m + bv[s][1] - b2m[s] + bv[s][2];
+                        db = dsum + bsum;
                     }
-                    dbn[s] = sb_db<DIM, I>(a, c, cim, cjm, ckm);
-                    fn[s] = ldx<T>(a.f, c);
-                    const T* U = a.Uin;
-                    T us = ldx<T>(U, cip) + ldx<T>(U, cim) + ldx<T>(U, cjp) + ldx<T>(U, cjm);
-                    if (DIM == 3) us = us + ldx<T>(U, ckp) + ldx<T>(U, ckm);
-                    U1p[zi & 3][by[s]][bx[s]] = sb_combine(a, dbn[s], us, fn[s]);
+                    dbn[s] = db;
+                    const T uxp = xp_in ? U0s[by[s]][sxp[s]] : ldx<T>(a.Uin, cip);
+                    const T uxm = xm_in ? U0s[by[s]][sxm[s]] : ldx<T>(a.Uin, cim);
+                    const T uyp = yp_in ? U0s[syp[s]][bx[s]] : ldx<T>(a.Uin, cjp);
+                    const T uym = ym_in ? U0s[sym[s]][bx[s]] : ldx<T>(a.Uin, cjm);
+                    T us = uxp + uxm + uyp + uym;
+                    if (DIM == 3) us = us + u0p[s] + u0m[s];
+                    U1p[zi & 3][by[s]][bx[s]] = sb_combine(a, db, us, fn[s]);
                 }
             }
         }
@@ -378,7 +431,16 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         }
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            if (s1) { dbz[s] = dbn[s]; fz[s] = fn[s]; }
+            if (s1) {
+                dbz[s] = dbn[s];
+                fz[s] = fn[s];
+                if (DIM == 3) {
+                    u0m[s] = u0c[s];
+                    u0c[s] = u0p[s];
+                    d2m[s] = dv[s][2];
+                    b2m[s] = bv[s][2];
+                }
+            }
         }
     }
     if (a.check) {
