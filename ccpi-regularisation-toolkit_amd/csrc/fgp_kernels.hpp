@@ -287,6 +287,8 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
 
     __shared__ T Dk[2][BH + 1][BW + 1];    // D_k, planes by parity
     __shared__ T Dk1[2][BH + 1][BW + 1];   // D_{k+1}
+    __shared__ T Rk1[BH + 1][BW + 1];      // R_k,1 and ,2 of plane zi
+    __shared__ T Rk2[BH + 1][BW + 1];
     __shared__ T R1s[BH + 1][BW + 1];      // R_{k+1},1 and ,2 of the current plane
     __shared__ T R2s[BH + 1][BW + 1];
     __shared__ double red[8];
@@ -315,7 +317,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     const int zstart = z0 >= 1 ? z0 - 1 : 0;
     const int zend = z1 + 1;
     for (int zi = zstart; zi <= zend; ++zi) {
-        // ---- S1: D_k(zi)
+        // ---- S1a: R_k(zi) on the region, into LDS
         T rn[2][3] = {}, pn[2][3] = {}, fn[2] = {};
         const bool s1 = zi < nz;
         if (s1) {
@@ -323,17 +325,41 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
             for (int s = 0; s < 2; ++s) {
                 if (s < nslot) {
                     const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
-                    const I im = cx[s] > 0 ? i - 1 : i, jm = cy[s] > 0 ? i - nx : i;
                     #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const T p = ldx<T>(a.P[c], i), q = ldx<T>(a.Pm[c], i);
                         pn[s][c] = p;
                         rn[s][c] = p + m0 * (p - q);
                     }
-                    const T b1p = ldx<T>(a.P[0], im), b1q = ldx<T>(a.Pm[0], im);
-                    const T b2p = ldx<T>(a.P[1], jm), b2q = ldx<T>(a.Pm[1], jm);
-                    const T b1r = b1p + m0 * (b1p - b1q), b2r = b2p + m0 * (b2p - b2q);
                     fn[s] = ldx<T>(a.f, i);
+                    Rk1[by[s]][bx[s]] = rn[s][0];
+                    Rk2[by[s]][bx[s]] = rn[s][1];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- S1b: D_k(zi); the -x / -y neighbours of the region's first
+        // column / row lie outside it and are evaluated from memory
+        if (s1) {
+            #pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s < nslot) {
+                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    T b1r, b2r;
+                    if (bx[s] > 0) {
+                        b1r = Rk1[by[s]][bx[s] - 1];
+                    } else {
+                        const I im = cx[s] > 0 ? i - 1 : i;
+                        const T b1p = ldx<T>(a.P[0], im), b1q = ldx<T>(a.Pm[0], im);
+                        b1r = b1p + m0 * (b1p - b1q);
+                    }
+                    if (by[s] > 0) {
+                        b2r = Rk2[by[s] - 1][bx[s]];
+                    } else {
+                        const I jm = cy[s] > 0 ? i - nx : i;
+                        const T b2p = ldx<T>(a.P[1], jm), b2q = ldx<T>(a.Pm[1], jm);
+                        b2r = b2p + m0 * (b2p - b2q);
+                    }
                     const T b1 = X[s] > 0 ? b1r : (T)0;
                     const T b2 = Y[s] > 0 ? b2r : (T)0;
                     const T b3 = zi > 0 ? rk[s][2] : (T)0;
