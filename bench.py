@@ -29,7 +29,7 @@ Also reported:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu] [--no-batch]
                        [--no-serve] [--serve-window SECONDS]
        python bench.py --profile-pass --steps K --warmup 0   (the headline solves only)
-       python bench.py --workload {tgv3d,pd3d,sb3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
+       python bench.py --workload {tgv3d,pd3d,sb3d,rof3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
        profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
        known size that calibrates the FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -477,6 +477,13 @@ def single_workload(name, steps, D, torch):
         D.pd(x, 0.02, steps, 0.0, 12.0, 0, 1, profile=True)
         m, _, c = D.last_profile()
         return {"workload": "PD-TV 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c}
+    if name == "rof3d":
+        x = torch.from_numpy(fgp_workload()).cuda()
+        D.rof(x, 0.02, 10, 0.001, 0.0)
+        D.rof(x, 0.02, steps, 0.001, 0.0, profile=True)
+        m, _, c = D.last_profile()
+        return {"workload": "ROF-TV 3D 256^3", "kernel_ms_mean_per_iteration": m, "iterations_timed": c,
+                "voxel_iters_per_s_kernel": x.numel() / (m * 1e-3) if m else None}
     if name == "sb3d":
         x = torch.from_numpy(fgp_workload()).cuda()
         D.sb(x, 0.02, steps, 0.0, 0, profile=True)
@@ -513,7 +520,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-window", type=float, default=0.5,
                     help="seconds of Poisson arrivals replayed by the served section")
-    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "sb3d", "dtv3d", "llt3d",
+    ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "sb3d", "rof3d", "dtv3d", "llt3d",
                                                                 "ndf3d", "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
     if args.gpus != 1:

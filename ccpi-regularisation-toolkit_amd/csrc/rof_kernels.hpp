@@ -70,69 +70,116 @@ __device__ inline void rof_fluxes(const RofArgs<T>& a, int px, int py, int z, T&
 template <int DIM, int TY, typename T, typename I>
 __device__ inline void rof_body(const RofArgs<T>& a, const TileCoord& tc) {
     const Geom& g = a.g;
-    constexpr int NT = TX * TY, NH = TX + TY;
+    constexpr int NT = TX * TY;
+    // U plane ring: rows y0-1 and y0+TY on [x0-1, x0+64] (corners included),
+    // columns x0-1 and x0+64 on the tile's rows
+    constexpr int NR = 2 * (TX + 2) + 2 * TY;
+    static_assert(NR <= NT, "ring lanes must fit the workgroup");
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
     const int x = tc.x0 + lx, y = tc.y0 + ly;
     const bool own = x < g.nx && y < g.ny;
-    // -x halo column (D2, TY points) and -y halo row (D1, TX points) on the
-    // low or high waves depending on the block index
-    const int hbase = (tc.lin & 1) ? NT - NH : 0;
-    const int h = tid - hbase;
-    const bool hlane = h >= 0 && h < NH;
-    const bool hcol = h < TY;
-    const int hx = hcol ? tc.x0 - 1 : tc.x0 + h - TY;
-    const int hy = hcol ? tc.y0 + h : tc.y0 - 1;
-    const bool hown = hlane && hx >= 0 && hy >= 0 && hx < g.nx && hy < g.ny;
     const int ox = own ? x : 0, oy = own ? y : 0;
-    const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
+    // ring lane: one point of the U ring, on the low or high waves by block index
+    const int hbase = (tc.lin & 1) ? NT - NR : 0;
+    const int h = tid - hbase;
+    const bool rlane = h >= 0 && h < NR;
+    int sx, sy;   // ring slot in the (TY+2) x (TX+2) plane with origin (x0-1, y0-1)
+    if (h < TX + 2) { sx = h; sy = 0; }
+    else if (h < 2 * (TX + 2)) { sx = h - (TX + 2); sy = TY + 1; }
+    else if (h < 2 * (TX + 2) + TY) { sx = 0; sy = 1 + h - 2 * (TX + 2); }
+    else { sx = TX + 1; sy = 1 + h - 2 * (TX + 2) - TY; }
+    const int rx = tc.x0 + sx - 1, ry = tc.y0 + sy - 1;
+    const bool rin = rlane && rx >= 0 && ry >= 0 && rx < g.nx && ry < g.ny;
+    const int rcx = rin ? rx : 0, rcy = rin ? ry : 0;   // clamped stand-in
+    // ring lanes that also evaluate a flux: D1 on row y0-1 over x0..x0+63 and
+    // D2 on column x0-1 over the tile's rows
+    const bool d1lane = rlane && sy == 0 && sx >= 1 && sx <= TX;
+    const bool d2lane = rlane && sx == 0 && sy >= 1 && sy <= TY;
 
+    __shared__ T Us[TY + 2][TX + 2];    // U of plane z: tile and ring
     __shared__ T D1l[TY + 1][TX];       // [ly + 1][lx]; row 0 = y0 - 1
     __shared__ T D2l[TY][TX + 1];       // [ly][lx + 1]; column 0 = x0 - 1
     __shared__ double red[8];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
 
+    // mirrored neighbour slots (loop-invariant) of the own point; threads
+    // outside the volume read their own slot (their results are not stored)
+    const int oxp = own ? mirror_p(x, g.nx) - tc.x0 + 1 : lx + 1, oxm = own ? mirror_m(x, g.nx) - tc.x0 + 1 : lx + 1;
+    const int oyp = own ? mirror_p(y, g.ny) - tc.y0 + 1 : ly + 1, oym = own ? mirror_m(y, g.ny) - tc.y0 + 1 : ly + 1;
+    // flux lanes: the neighbours a D1 on row y0-1 / a D2 on column x0-1 reads
+    const int rxp = mirror_p(rcx, g.nx) - tc.x0 + 1, rxm = mirror_m(rcx, g.nx) - tc.x0 + 1;
+    const int ryp = mirror_p(rcy, g.ny) - tc.y0 + 1, rym = mirror_m(rcy, g.ny) - tc.y0 + 1;
+
+    const I ocol = (I)oy * nx + (I)ox, rcol = (I)rcy * nx + (I)rcx;
+    // z pipeline of the own column and of the ring point
+    T uc = ldx<T>(a.U, (I)tc.z0 * sxy + ocol), um = 0;
+    T hc = 0, hm = 0;
+    if (rlane) hc = ldx<T>(a.U, (I)tc.z0 * sxy + rcol);
     // D3 at the mirrored z-predecessor of the chunk's first plane
     T d3m = 0;
-    if (DIM == 3 && g.nz > 1) {
-        T t1, t2;
-        rof_fluxes<DIM, I>(a, ox, oy, mirror_m(tc.z0, g.nz), t1, t2, d3m);
+    if (DIM == 3) {
+        const int zm = mirror_m(tc.z0, g.nz);
+        um = ldx<T>(a.U, (I)zm * sxy + ocol);
+        if (rlane) hm = ldx<T>(a.U, (I)zm * sxy + rcol);
+        if (g.nz > 1) {
+            T t1, t2;
+            rof_fluxes<DIM, I>(a, ox, oy, zm, t1, t2, d3m);
+        }
     }
     double sdd = 0.0, suu = 0.0;
     for (int z = tc.z0; z < tc.z1; ++z) {
+        T up = 0, hp = 0;
+        if (DIM == 3) {
+            const int zp = mirror_p(z, g.nz);
+            up = (zp == z + 1) ? ldx<T>(a.U, (I)zp * sxy + ocol) : um;
+            if (rlane) hp = (zp == z + 1) ? ldx<T>(a.U, (I)zp * sxy + rcol) : hm;
+        }
+        Us[ly + 1][lx + 1] = uc;
+        if (rlane) Us[sy][sx] = hc;
+        __syncthreads();
+        // fluxes of the own point
         T d1, d2, d3;
-        rof_fluxes<DIM, I>(a, ox, oy, z, d1, d2, d3);
+        rof_flux_math<DIM, T>(uc, Us[ly + 1][oxp], Us[ly + 1][oxm], Us[oyp][lx + 1], Us[oym][lx + 1], up, um, d1, d2,
+                              d3);
         D1l[ly + 1][lx] = d1;
         D2l[ly][lx + 1] = d2;
-        if (hlane) {
+        if (d1lane || d2lane) {
+            // the neighbour outside the ring (y0-2 for D1, x0-2 for D2) feeds
+            // only the other fluxes; it is replaced by the point itself
+            const T uxp = Us[sy][rxp >= 0 && rxp < TX + 2 ? rxp : sx];
+            const T uxm = Us[sy][rxm >= 0 && rxm < TX + 2 ? rxm : sx];
+            const T uyp = Us[ryp >= 0 && ryp < TY + 2 ? ryp : sy][sx];
+            const T uym = Us[rym >= 0 && rym < TY + 2 ? rym : sy][sx];
             T h1, h2, h3;
-            rof_fluxes<DIM, I>(a, hcx, hcy, z, h1, h2, h3);
-            if (hown) {
-                if (hcol) D2l[h][0] = h2;
-                else D1l[0][h - TY] = h1;
+            rof_flux_math<DIM, T>(hc, uxp, uxm, uyp, uym, hp, hm, h1, h2, h3);
+            if (rin) {
+                if (d2lane) D2l[sy - 1][0] = h2;
+                else D1l[0][sx - 1] = h1;
             }
         }
         __syncthreads();
         if (own) {
-            const I i = (I)z * sxy + (I)y * nx + (I)x;
-            const int jm = mirror_m(y, g.ny), im = mirror_m(x, g.nx);
-            const T dv1 = d1 - D1l[jm - tc.y0 + 1][lx];
-            const T dv2 = d2 - D2l[ly][im - tc.x0 + 1];
+            const I i = (I)z * sxy + ocol;
+            const T dv1 = d1 - D1l[oym][lx];
+            const T dv2 = d2 - D2l[ly][oxm];
             T dv = dv1 + dv2;
             if (DIM == 3) {
                 // at z = 0 the mirrored predecessor is plane 1 (d3m from the prologue)
                 dv = dv + (d3 - d3m);
             }
-            const T u = ldx<T>(a.U, i);
-            const T un = u + a.tau * (a.lambda * dv - (u - ldx<T>(a.f, i)));
+            const T un = uc + a.tau * (a.lambda * dv - (uc - ldx<T>(a.f, i)));
             if (a.check) {
-                const T df = un - u;
+                const T df = un - uc;
                 sdd += (double)df * (double)df;
                 suu += (double)un * (double)un;
             }
             stnt<T>(a.Uout, i, un);
         }
-        __syncthreads();
         d3m = d3;
+        um = uc;
+        uc = up;
+        hm = hc;
+        hc = hp;
     }
     if (a.check) {
         block_sum2(sdd, suu, red);
