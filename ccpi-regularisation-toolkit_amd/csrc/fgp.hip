@@ -43,32 +43,32 @@ static void launch_fgp(const FgpArgs<T, S>& a, bool iso, bool small, hipStream_t
 // Device-resident FGP solve.  d_out receives D of the final iteration (left
 // untouched when iter == 0); with epsil != 0 its prior contents seed the first
 // residual (FGP_TV_core.c:66,123).
-template <int TY, typename T, typename S>
+template <int TY, typename T, typename S, int DIM = 3>
 static const void* fgp2_kernel_ptr(bool iso, bool small) {
     if (small)
-        return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint32_t>
-                   : (const void*)fgp2_iter_kernel<TY, T, S, false, uint32_t>;
-    return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint64_t>
-               : (const void*)fgp2_iter_kernel<TY, T, S, false, uint64_t>;
+        return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint32_t, DIM>
+                   : (const void*)fgp2_iter_kernel<TY, T, S, false, uint32_t, DIM>;
+    return iso ? (const void*)fgp2_iter_kernel<TY, T, S, true, uint64_t, DIM>
+               : (const void*)fgp2_iter_kernel<TY, T, S, false, uint64_t, DIM>;
 }
 
-template <int TY, typename T, typename S>
+template <int TY, typename T, typename S, int DIM = 3>
 static void launch_fgp2(const Fgp2Args<T, S>& a, bool iso, bool small, hipStream_t s) {
     dim3 block(TX, TY), grid(a.g.nblocks);
     if (small) {
-        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint32_t>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint32_t>), grid, block, 0, s, a);
+        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint32_t, DIM>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint32_t, DIM>), grid, block, 0, s, a);
     } else {
-        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint64_t>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint64_t>), grid, block, 0, s, a);
+        if (iso) hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, true, uint64_t, DIM>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fgp2_iter_kernel<TY, T, S, false, uint64_t, DIM>), grid, block, 0, s, a);
     }
 }
 
 // Device-resident FGP solve.  d_out receives D of the final iteration (left
 // untouched when iter == 0); with epsil != 0 its prior contents seed the first
-// residual (FGP_TV_core.c:66,123).  In 3D, iterations run two per pass
-// (fgp2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
-// 2D iteration, runs on the one-iteration kernel.  P_j lives in buffer
+// residual (FGP_TV_core.c:66,123).  Iterations run two per pass
+// (fgp2_iter_kernel) unless opts.depth == 1; an odd last iteration runs on the
+// one-iteration kernel.  P_j lives in buffer
 // j mod nbuf (nbuf = 4 with two-iteration passes, which write P_{k+1} and
 // P_{k+2} while reading P_k and P_{k-1}; 3 otherwise).
 template <typename T, typename S>
@@ -84,9 +84,13 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
     // returning the state after a stop needs the stop iteration's own pass
-    const int width = (dim == 3 && opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
+    const int width = (opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
+    // 2D two-iteration passes: 64 x 8 tiles unless 4 rows are asked for
+    const int ty2d = opts.tile_y == 4 ? 4 : 8;
     Geom g;
-    if (dim == 2)
+    if (dim == 2 && width == 2)
+        g = make_geom(nx, ny, nz, ty2d, 1);
+    else if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
     else if (width == 2)
         // the pass evaluates about three planes below and above its chunk
@@ -177,8 +181,14 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
             a.partials1 = partials1;
             a.ctrl = ctrl;
             a.g = g;
-            if (ty == 8) launch_fgp2<8>(a, iso, small, s);
-            else launch_fgp2<4>(a, iso, small, s);
+            if (dim == 2) {
+                if (ty2d == 8) launch_fgp2<8, T, S, 2>(a, iso, small, s);
+                else launch_fgp2<4, T, S, 2>(a, iso, small, s);
+            } else if (ty == 8) {
+                launch_fgp2<8>(a, iso, small, s);
+            } else {
+                launch_fgp2<4>(a, iso, small, s);
+            }
             return;
         }
         FgpArgs<T, S> a;
@@ -202,6 +212,8 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
         if (dim == 3) {
             if (ty == 8) launch_fgp<3, 8>(a, iso, small, s);
             else launch_fgp<3, 4>(a, iso, small, s);
+        } else if (width == 2 && ty2d == 8) {
+            launch_fgp<2, 8>(a, iso, small, s);   // odd last iteration on the pass geometry
         } else {
             launch_fgp<2, 4>(a, iso, small, s);
         }

@@ -245,7 +245,8 @@ struct Fgp2Shape {
     static constexpr int NT = TX * TY;
 };
 
-template <int TY, typename T, typename S, bool ISO, typename I>
+// DIM = 2 runs the same pipeline on one plane (no z terms; P has two components).
+template <int TY, typename T, typename S, bool ISO, typename I, int DIM = 3>
 __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     using Sh = Fgp2Shape<TY>;
     constexpr int BW = Sh::BW, BH = Sh::BH, NR = Sh::NR, NT = Sh::NT;
@@ -303,7 +304,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     T rq[2][3] = {};    // R_{k+1}(zi - 2)
     T fz[2] = {};       // f(zi - 1)
 
-    if (z0 >= 2) {   // R_k,3 two planes below the chunk start
+    if (DIM == 3 && z0 >= 2) {   // R_k,3 two planes below the chunk start
         #pragma unroll
         for (int s = 0; s < 2; ++s) {
             if (s < nslot) {
@@ -326,7 +327,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                 if (s < nslot) {
                     const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
                     #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
+                    for (int c = 0; c < DIM; ++c) {
                         const T p = ldx<T>(a.P[c], i), q = ldx<T>(a.Pm[c], i);
                         pn[s][c] = p;
                         rn[s][c] = p + m0 * (p - q);
@@ -362,8 +363,13 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     }
                     const T b1 = X[s] > 0 ? b1r : (T)0;
                     const T b2 = Y[s] > 0 ? b2r : (T)0;
-                    const T b3 = zi > 0 ? rk[s][2] : (T)0;
-                    T d = fn[s] - lam * (rn[s][0] + rn[s][1] + rn[s][2] - b1 - b2 - b3);
+                    T d;
+                    if (DIM == 3) {
+                        const T b3 = zi > 0 ? rk[s][2] : (T)0;
+                        d = fn[s] - lam * (rn[s][0] + rn[s][1] + rn[s][2] - b1 - b2 - b3);
+                    } else {
+                        d = fn[s] - lam * (rn[s][0] + rn[s][1] - b1 - b2);
+                    }
                     if (a.nonneg && d < (T)0) d = (T)0;
                     Dk[zi & 1][by[s]][bx[s]] = d;
                     if (s == 0 && own && zi >= z0 && zi < z1) {
@@ -391,23 +397,28 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     const T d = Dk[b][by[s]][bx[s]];
                     const T g1 = (X[s] == g.nx - 1) ? (T)0 : d - Dk[b][by[s]][bx[s] + 1];
                     const T g2 = (Y[s] == g.ny - 1) ? (T)0 : d - Dk[b][by[s] + 1][bx[s]];
-                    const T g3 = (zc == nz - 1) ? (T)0 : d - Dk[b ^ 1][by[s]][bx[s]];
                     T p1 = rk[s][0] + step * g1;
                     T p2 = rk[s][1] + step * g2;
-                    T p3 = rk[s][2] + step * g3;
-                    fgp_project<3, T, ISO>(p1, p2, p3);
+                    T p3 = 0;
+                    if (DIM == 3) {
+                        const T g3 = (zc == nz - 1) ? (T)0 : d - Dk[b ^ 1][by[s]][bx[s]];
+                        p3 = rk[s][2] + step * g3;
+                    }
+                    fgp_project<DIM, T, ISO>(p1, p2, p3);
                     if (s == 0 && own && zc >= z0 && zc < z1) {
                         const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
                         stnt<T>(a.Pn[0], io, p1);
                         stnt<T>(a.Pn[1], io, p2);
-                        stnt<T>(a.Pn[2], io, p3);
+                        if (DIM == 3) stnt<T>(a.Pn[2], io, p3);
                     }
                     p1 = through_storage<T, S>(p1);
                     p2 = through_storage<T, S>(p2);
-                    p3 = through_storage<T, S>(p3);
                     rqn[s][0] = p1 + m1 * (p1 - pk[s][0]);
                     rqn[s][1] = p2 + m1 * (p2 - pk[s][1]);
-                    rqn[s][2] = p3 + m1 * (p3 - pk[s][2]);
+                    if (DIM == 3) {
+                        p3 = through_storage<T, S>(p3);
+                        rqn[s][2] = p3 + m1 * (p3 - pk[s][2]);
+                    }
                     R1s[by[s]][bx[s]] = rqn[s][0];
                     R2s[by[s]][bx[s]] = rqn[s][1];
                 }
@@ -424,8 +435,13 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     const T b2r = R2s[by[s] > 0 ? by[s] - 1 : 0][bx[s]];
                     const T b1 = X[s] > 0 ? b1r : (T)0;
                     const T b2 = Y[s] > 0 ? b2r : (T)0;
-                    const T b3 = zc > 0 ? rq[s][2] : (T)0;
-                    T d = fz[s] - lam * (rqn[s][0] + rqn[s][1] + rqn[s][2] - b1 - b2 - b3);
+                    T d;
+                    if (DIM == 3) {
+                        const T b3 = zc > 0 ? rq[s][2] : (T)0;
+                        d = fz[s] - lam * (rqn[s][0] + rqn[s][1] + rqn[s][2] - b1 - b2 - b3);
+                    } else {
+                        d = fz[s] - lam * (rqn[s][0] + rqn[s][1] - b1 - b2);
+                    }
                     if (a.nonneg && d < (T)0) d = (T)0;
                     Dk1[zc & 1][by[s]][bx[s]] = d;
                     if (s == 0 && own && zc < z1) {
@@ -448,15 +464,18 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
             const T d = Dk1[b][by[0]][bx[0]];
             const T g1 = (X[0] == g.nx - 1) ? (T)0 : d - Dk1[b][by[0]][bx[0] + 1];
             const T g2 = (Y[0] == g.ny - 1) ? (T)0 : d - Dk1[b][by[0] + 1][bx[0]];
-            const T g3 = (zq == nz - 1) ? (T)0 : d - Dk1[b ^ 1][by[0]][bx[0]];
             T p1 = rq[0][0] + step * g1;
             T p2 = rq[0][1] + step * g2;
-            T p3 = rq[0][2] + step * g3;
-            fgp_project<3, T, ISO>(p1, p2, p3);
+            T p3 = 0;
+            if (DIM == 3) {
+                const T g3 = (zq == nz - 1) ? (T)0 : d - Dk1[b ^ 1][by[0]][bx[0]];
+                p3 = rq[0][2] + step * g3;
+            }
+            fgp_project<DIM, T, ISO>(p1, p2, p3);
             const I io = (I)zq * sxy + (I)Y[0] * nx + (I)X[0];
             stnt<T>(a.Pnn[0], io, p1);
             stnt<T>(a.Pnn[1], io, p2);
-            stnt<T>(a.Pnn[2], io, p3);
+            if (DIM == 3) stnt<T>(a.Pnn[2], io, p3);
         }
         // ---- shift the pipeline
         #pragma unroll
@@ -486,10 +505,10 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     }
 }
 
-template <int TY, typename T, typename S, bool ISO, typename I>
+template <int TY, typename T, typename S, bool ISO, typename I, int DIM = 3>
 __global__ __launch_bounds__(TX * TY) void fgp2_iter_kernel(Fgp2Args<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
-    fgp2_body<TY, T, S, ISO, I>(a, tile_of_block(a.g, TY));
+    fgp2_body<TY, T, S, ISO, I, DIM>(a, tile_of_block(a.g, TY));
 }
 
 }  // namespace cilreg

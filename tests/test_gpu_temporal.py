@@ -73,6 +73,45 @@ def test_depth2_wide_index():
     _same(a, b, "wide index")
 
 
+# 2D: the same pipeline on one plane (P has two components, no z terms).
+SHAPES_2D = [(70, 130), (1, 65), (33, 2), (257, 200), (3, 3), (9, 64)]
+
+
+@pytest.mark.parametrize("shape", SHAPES_2D, ids=lambda s: "x".join(map(str, s)))
+def test_depth2_matches_depth1_2d_tilings(shape):
+    x = _vol(shape, 15)
+    for tile_y in (4, 8):
+        for iters in (1, 2, 7, 10):
+            a, b = _pair(x, lam=0.03, iters=iters, tile_y=tile_y, nonneg=1)
+            _same(a, b, f"{shape} tile_y={tile_y} iters={iters}")
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["iso", "aniso"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_depth2_matches_depth1_2d_methods(method, precision):
+    x = _vol((150, 190), 16)
+    for nonneg in (0, 1):
+        a, b = _pair(x, lam=0.02, iters=25, method=method, nonneg=nonneg, precision=precision)
+        _same(a, b, f"2D method={method} nonneg={nonneg} {precision}")
+
+
+@pytest.mark.parametrize("check_every", [1, 2, 3, 5])
+def test_depth2_matches_depth1_2d_tolerance_stops(check_every):
+    x = _vol((200, 240), 17)
+    stopped = False
+    for tol in (1e-2, 3e-3, 1e-4):
+        a, b = _pair(x, lam=0.03, iters=120, tol=tol, check_every=check_every)
+        _same(a, b, f"2D tol={tol} check_every={check_every}")
+        stopped |= int(a[1][0]) < 120
+    assert stopped
+
+
+def test_depth2_2d_wide_index():
+    x = _vol((90, 130), 18)
+    a, b = _pair(x, lam=0.03, iters=9, wide_index=True)
+    _same(a, b, "2D wide index")
+
+
 # ---------------------------------------------------------------------------
 # TGV: the two-iteration pass reads the state once and writes it once; the
 # intermediate fields are rounded through the storage type as the
