@@ -198,10 +198,10 @@ __global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* 
     fgp_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
 }
 
-// 3D FGP groups advance two levels per launch: a job at an even iteration
+// FGP groups advance two levels per launch: a job at an even iteration
 // index k runs the two-iteration pass, or at its last (odd-count) iteration
 // the one-iteration body.  Jobs start at even levels, so k is always even.
-template <int TY, typename T, typename S, bool ISO, typename I>
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
 __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total,
                                                              int level) {
     const FgpJob<T, S>* j;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
         a.partials1 = uni(j->partials1);
         a.ctrl = uni(j->h.ctrl);
         a.g = uni_geom(j->h.g);
-        fgp2_body<TY, T, S, ISO, I>(a, tc);
+        fgp2_body<TY, T, S, ISO, I, DIM>(a, tc);
     } else {
         FgpArgs<T, S> a;
         #pragma unroll
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
         a.partials = uni(j->h.partials);
         a.ctrl = uni(j->h.ctrl);
         a.g = uni_geom(j->h.g);
-        fgp_body<3, TY, T, S, ISO, I>(a, tc);
+        fgp_body<DIM, TY, T, S, ISO, I>(a, tc);
     }
 }
 
@@ -554,8 +554,8 @@ struct EJob {
     long long ticket = 0;
     int state = J_PENDING;
     int dim = 3, nz = 1, ty = 8;
-    int nbuf = 3;   // FGP dual sets: 4 with two-iteration passes (3D)
-    int width = 1;  // iterations per launch (2: 3D FGP / TGV two-iteration passes)
+    int nbuf = 3;   // FGP dual sets: 4 with two-iteration passes
+    int width = 1;  // iterations per launch (2: two-iteration FGP / PD / TGV passes)
     bool wide = false;
     size_t n = 0, bytes = 0;
     std::unique_ptr<Arena> ar;   // every device buffer of the job, one allocation
@@ -575,7 +575,7 @@ struct EJob {
 };
 
 // group key: method, dimensionality, projection, index width, iterations
-// per launch (2 for 3D FGP / TGV unless the autotuner chose 1)
+// per launch (2 for two-iteration passes unless the autotuner chose 1)
 struct GroupKey {
     int method, dim, iso, wide, width;
     bool operator<(const GroupKey& o) const {
@@ -623,7 +623,9 @@ public:
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
         j->ty = j->dim == 3 ? 8 : 4;
-        const bool two = j->pub.method != CILREG_ROF && j->dim == 3;
+        // two-iteration passes: FGP, PD and TGV in 3D, FGP on small 2D images
+        const bool two = (j->pub.method != CILREG_ROF && j->dim == 3) ||
+                         (j->pub.method == CILREG_FGP && j->n < kFgp2dPassMaxPixels);
         int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
         j->width = two && opts_.depth != 1 ? 2 : 1;
         if (opts_.autotune && j->dim == 3 && opts_.zchunk == 0 && opts_.depth == 0) {
@@ -1055,21 +1057,13 @@ private:
                 break;
             case CILREG_FGP: {
                 auto* jp = (const FgpJob<T, S>*)t.dev;
-                if constexpr (DIM == 3) {
-                    if (k.width == 1) {
-                        if (w) {
-                            if (iso) hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                            else hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                        } else {
-                            if (iso) hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                            else hipLaunchKernelGGL((fgp_group_kernel<3, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                        }
-                    } else if (w) {
-                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                        else hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                if (k.width == 2) {
+                    if (w) {
+                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        else hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
                     } else {
-                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                        else hipLaunchKernelGGL((fgp2_group_kernel<TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        else hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
                     }
                 } else if (w) {
                     if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
@@ -1147,7 +1141,7 @@ private:
             const GroupKey& k = groups[gi].first;
             const Table& t = *groups[gi].second;
             hipStream_t st = side_[gi];
-            // 3D FGP and TGV groups advance two levels per launch (l0 is even)
+            // groups of two-iteration passes advance two levels per launch (l0 is even)
             const int width = k.width;
             for (int lv = l0; lv < l1; lv += width) {
                 int nact = 0;

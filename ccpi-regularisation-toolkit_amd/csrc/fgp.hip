@@ -67,7 +67,8 @@ static void launch_fgp2(const Fgp2Args<T, S>& a, bool iso, bool small, hipStream
 // Device-resident FGP solve.  d_out receives D of the final iteration (left
 // untouched when iter == 0); with epsil != 0 its prior contents seed the first
 // residual (FGP_TV_core.c:66,123).  Iterations run two per pass
-// (fgp2_iter_kernel) unless opts.depth == 1; an odd last iteration runs on the
+// (fgp2_iter_kernel) unless opts.depth == 1 or, by default, a 2D image has
+// kFgp2dPassMaxPixels or more; an odd last iteration runs on the
 // one-iteration kernel.  P_j lives in buffer
 // j mod nbuf (nbuf = 4 with two-iteration passes, which write P_{k+1} and
 // P_{k+2} while reading P_k and P_{k-1}; 3 otherwise).
@@ -84,9 +85,10 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     const bool iso = method == 0;
     // returning the state after a stop needs the stop iteration's own pass
-    const int width = (opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
-    // 2D two-iteration passes: 64 x 8 tiles unless 4 rows are asked for
-    const int ty2d = opts.tile_y == 4 ? 4 : 8;
+    const bool pass2 = opts.depth == 2 || (opts.depth == 0 && (dim == 3 || n < kFgp2dPassMaxPixels));
+    const int width = (pass2 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
+    // 2D two-iteration passes: 64 x 4 tiles unless 8 rows are asked for
+    const int ty2d = opts.tile_y == 8 ? 8 : 4;
     Geom g;
     if (dim == 2 && width == 2)
         g = make_geom(nx, ny, nz, ty2d, 1);

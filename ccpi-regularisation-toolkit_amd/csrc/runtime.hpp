@@ -130,9 +130,15 @@ struct Opts {
     int profile = 0;       // time each iteration with HIP events
     int wide_index = 0;    // force 64-bit element indexing
     int stream_launches = 0;  // batch: launch directly instead of one hipGraph
-    int depth = 0;         // iterations per pass where supported (FGP, PD, TGV 3D: 2 by default; 1 = off)
+    int depth = 0;         // iterations per pass where supported (FGP, PD, TGV 3D; FGP 2D below 2^20 pixels: 2 by default; 1 = off)
     int autotune = 0;      // choose tile_y / zchunk / depth by timing (tune.hip) when none is given
 };
+
+// 2D FGP runs two iterations per pass below this many pixels: there the
+// one-iteration kernel is launch/latency-bound (256^2: 6.5 -> 3.8 us per
+// iteration); from 1024^2 on the pass's ring evaluation costs more than the
+// traffic it saves (2048^2: 30.5 vs 32.3 us).
+constexpr size_t kFgp2dPassMaxPixels = (size_t)1 << 20;
 
 // Per-iteration kernel timing of the last profiled solve on this thread.
 struct ProfileStats { double mean_ms = 0, min_ms = 0; int count = 0, passes = 0; };

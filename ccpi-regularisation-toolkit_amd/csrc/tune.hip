@@ -67,8 +67,9 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
         set_error("autotune: unknown method %d", method);
         return -1;
     }
-    if (nz <= 1) {   // 2D: one plane per workgroup, 64 x 4 tiles, one iteration per pass
-        *out = TuneChoice{4, 1, 1, 0.0};
+    if (nz <= 1) {   // 2D: one plane per workgroup, 64 x 4 tiles; small FGP images run two iterations per pass
+        const bool two = method == CILREG_FGP && (size_t)nx * ny < kFgp2dPassMaxPixels;
+        *out = TuneChoice{4, 1, two ? 2 : 1, 0.0};
         return 0;
     }
     const Key key{method, base.precision, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX, (ny + 31) / 32 * 32,

@@ -1,9 +1,10 @@
 """Two-iteration FGP, PD, TGV and FGP-dTV passes (temporal blocking) against
 one-iteration passes.
 
-The 3D FGP solver fuses iterations k and k+1 into one pass by default; the
-pass evaluates iteration k on a ring around its tile and every plane through
-a four-stage z pipeline.  Each value is computed by the same expressions in
+The FGP solver fuses iterations k and k+1 into one pass by default (3D, and
+2D images below 2^20 pixels; depth=2 forces it for any 2D image); the pass
+evaluates iteration k on a ring around its tile and every plane through a
+four-stage z pipeline.  Each value is computed by the same expressions in
 the same order as the one-iteration kernel, so the results and the
 infovector must be bit-identical to depth 1 (itself pinned to the oracle by
 test_gpu_parity.py) for every tiling, chunking (down to one plane per
@@ -74,7 +75,7 @@ def test_depth2_wide_index():
 
 
 # 2D: the same pipeline on one plane (P has two components, no z terms).
-SHAPES_2D = [(70, 130), (1, 65), (33, 2), (257, 200), (3, 3), (9, 64)]
+SHAPES_2D = [(70, 130), (1, 65), (33, 2), (257, 200), (3, 3), (9, 64), (1030, 1027)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_2D, ids=lambda s: "x".join(map(str, s)))
