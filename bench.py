@@ -14,7 +14,9 @@ Also reported:
   launch;
 * cpu_baseline: the reference C-OMP library (oracle/_ref, built from the
   reference sources) or, where it is absent, the C restatement (oracle/), on a
-  bounded sample of the same workload, on the host's cores;
+  bounded sample of the same workload, on the host's cores; with the extras,
+  also on bounded samples of every other measured workload
+  (cpu_baseline.other_workloads);
 * secondary: TGV 3D 192x512x512 (500 iterations; FP32, BF16 and FP16 dual
   storage against an FP64 solve), PD-TV 3D 256^3 (400 iterations, L = 12),
   ROF-TV 2D 512^2 (2000 iterations); 64 uneven volumes (half FGP, half ROF)
@@ -104,6 +106,56 @@ def cpu_baseline(x_np, iters):
     return {"value": vi, "unit": "voxel-iters/s", "cores": threads, "kind": kind,
             "sample": f"FGP-TV 3D 256^3, {iters} of 400 iterations (fixed count), {dt:.1f} s",
             "seconds": round(dt, 2)}
+
+
+def cpu_rows(x_np):
+    """The reference CPU path on a bounded sample of every other measured
+    workload (same volumes and parameters as the GPU rows, fewer iterations
+    except ROF 2D, which runs its full 2000): voxel-iterations per second."""
+    import numpy as np
+    import oracle
+    from ccpi.supp import phantoms
+    lib = oracle.reference()
+    kind = "reference"
+    if lib is None:
+        lib = oracle.cpu_ref()
+        kind = "port"
+    rows = {}
+
+    def timed(name, method, x, params, label):
+        t0 = time.perf_counter()
+        _, info = lib.run(method, x, params)
+        dt = time.perf_counter() - t0
+        rows[name] = {"voxel_iters_per_s": x.size * int(info[0]) / dt, "seconds": round(dt, 3),
+                      "sample": label}
+
+    n256 = "256^3"
+    timed("rof_tv_2d_512", "rof", phantoms.noisy(phantoms.shapes_2d((512, 512), 20, seed=1), 0.05, seed=101),
+          {"lambda": 0.02, "iters": 2000, "tau": 0.001, "tol": 0.0}, "ROF-TV 2D 512^2, all 2000 iterations")
+    timed("rof_tv_3d_256", "rof", x_np, {"lambda": 0.02, "iters": 16, "tau": 0.001, "tol": 0.0},
+          f"ROF-TV 3D {n256}, 16 iterations")
+    timed("pd_tv_3d_256", "pd", x_np, {"lambda": 0.02, "iters": 16, "tol": 0.0, "L": 12.0, "method": 0, "nonneg": 1},
+          f"PD-TV 3D {n256}, 16 of 400 iterations")
+    timed("sb_tv_3d_256", "sb", x_np, {"mu": 0.02, "iters": 8, "tol": 0.0, "method": 0},
+          f"SB-TV 3D {n256}, 8 of 150 iterations")
+    timed("fgp_dtv_3d_256", "dtv", x_np, {"ref": dtv_reference(), "lambda": 0.02, "iters": 8, "tol": 0.0,
+                                          "eta": 0.01, "method": 0, "nonneg": 1},
+          f"FGP-dTV 3D {n256}, 8 of 400 iterations")
+    for name in ("llt", "ndf", "d4"):
+        (a, b), kw, _ = DIFFUSION[name]
+        if name == "llt":
+            prm = {"lambda_rof": a, "lambda_llt": b, "iters": 16, "tau": kw["tau"], "tol": 0.0}
+        elif name == "ndf":   # the reference's own update order (in place)
+            prm = {"lambda": a, "sigma": b, "iters": 8, "tau": kw["tau"], "penalty": kw["penalty"], "tol": 0.0,
+                   "in_place": True}
+        else:
+            prm = {"lambda": a, "sigma": b, "iters": 16, "tau": kw["tau"], "tol": 0.0}
+        timed(f"{name}_3d_256", name, x_np, prm, f"{name} 3D {n256}, {prm['iters']} of 500 iterations")
+    xt = phantoms.noisy(phantoms.ramp_ellipsoids_3d((192, 512, 512), 30, seed=3), 0.03, seed=103)
+    timed("tgv_3d_192x512x512", "tgv", xt, {"lambda": 0.02, "alpha1": 1.0, "alpha0": 2.0, "iters": 4, "L2": 12.0,
+                                            "tol": 0.0}, "TGV 3D 192x512x512, 4 of 500 iterations")
+    del xt
+    return {"kind": kind, "rows": rows}
 
 
 def secondary(D, torch):
@@ -602,6 +654,9 @@ def main():
     if not args.no_cpu:
         log(f"bench: CPU baseline, {args.cpu_iters} FGP iterations")
         cpu = cpu_baseline(x_np, args.cpu_iters)
+        if not args.no_extras:
+            log("bench: CPU baseline of the other workloads")
+            cpu["other_workloads"] = cpu_rows(x_np)
     extras = None
     if not args.no_extras:
         del x
