@@ -14,12 +14,32 @@ using namespace cilreg;
 
 namespace {
 
+// Drains the stream on an early return, before the device buffers of the
+// call go back to the allocator cache (declared after the Arena, so it runs
+// first); a completed call has already synchronised.
+struct DrainOnError {
+    hipStream_t s;
+    bool done = false;
+    ~DrainOnError() {
+        if (!done) (void)hipStreamSynchronize(s);
+    }
+};
+
+bool host_ptrs_ok(const void* in, const void* out, const void* info) {
+    if (!in || !out || !info) {
+        set_error("null %s pointer", !in ? "input" : !out ? "output" : "infovector");
+        return false;
+    }
+    return true;
+}
+
 // Upload `in` (and optionally the caller's `out` contents), run `solve` on
 // device buffers, download the result into `out` unless `keep_out_on_zero_iter`
 // and iter == 0.
 template <class Solve>
 int host_call(int gpu_device, const float* in, float* out, float* info, size_t n, bool upload_out, bool skip_download,
               Solve&& solve) {
+    if (!host_ptrs_ok(in, out, info)) return -1;
     if (select_device(gpu_device) != 0) return -1;
     hipStream_t s = device_stream();
     if (!s) { set_error("cannot create a stream on device %d", gpu_device); return -1; }
@@ -27,6 +47,7 @@ int host_call(int gpu_device, const float* in, float* out, float* info, size_t n
     const size_t oin = ar.reserve(n * sizeof(float));
     const size_t oout = ar.reserve(n * sizeof(float));
     if (ar.allocate() != 0) return -1;
+    DrainOnError drain{s};
     float* d_in = (float*)ar.at(oin);
     float* d_out = (float*)ar.at(oout);
     CILREG_CK(hipMemcpyAsync(d_in, in, n * sizeof(float), hipMemcpyHostToDevice, s));
@@ -38,6 +59,7 @@ int host_call(int gpu_device, const float* in, float* out, float* info, size_t n
         CILREG_CK(hipMemcpyAsync(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
     }
     CILREG_CK(hipStreamSynchronize(s));
+    drain.done = true;
     info[0] = (float)done;
     info[1] = re;
     return 0;
@@ -48,6 +70,8 @@ int host_call(int gpu_device, const float* in, float* out, float* info, size_t n
 template <class Solve>
 int host_call_ref(int gpu_device, const float* in, const float* ref, float* out, float* info, size_t n,
                   bool upload_out, bool skip_download, Solve&& solve) {
+    if (!host_ptrs_ok(in, out, info)) return -1;
+    if (!ref) { set_error("null reference image pointer"); return -1; }
     if (select_device(gpu_device) != 0) return -1;
     hipStream_t s = device_stream();
     if (!s) { set_error("cannot create a stream on device %d", gpu_device); return -1; }
@@ -56,6 +80,7 @@ int host_call_ref(int gpu_device, const float* in, const float* ref, float* out,
     const size_t oref = ar.reserve(n * sizeof(float));
     const size_t oout = ar.reserve(n * sizeof(float));
     if (ar.allocate() != 0) return -1;
+    DrainOnError drain{s};
     float* d_in = (float*)ar.at(oin);
     float* d_ref = (float*)ar.at(oref);
     float* d_out = (float*)ar.at(oout);
@@ -67,6 +92,7 @@ int host_call_ref(int gpu_device, const float* in, const float* ref, float* out,
     if (solve(d_in, d_ref, d_out, s, &done, &re) != 0) return -1;
     if (!skip_download) CILREG_CK(hipMemcpyAsync(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
     CILREG_CK(hipStreamSynchronize(s));
+    drain.done = true;
     info[0] = (float)done;
     info[1] = re;
     return 0;
@@ -241,6 +267,7 @@ int Diffus4th_GPU_main(float* Input, float* Output, float* infovector, float lam
 // device-resident extensions
 // ---------------------------------------------------------------------------
 #define DEV_PROLOGUE(NX, NY, NZ, METHOD, ISO)                    \
+    if (!d_input || !d_output) { set_error("null device pointer"); return -1; } \
     if (!dims_ok(NX, NY, NZ)) return -1;                         \
     const Opts o = tuned_opts(METHOD, NX, NY, (NZ) <= 1 ? 1 : (NZ), ISO, resolve_opts(opts)); \
     hipStream_t s = pick_stream(stream);                         \
@@ -250,7 +277,7 @@ int Diffus4th_GPU_main(float* Input, float* Output, float* infovector, float lam
     int rc = -1;
 
 #define DEV_EPILOGUE()                        \
-    if (rc == 0) {                            \
+    if (rc == 0 && infovector) {              \
         infovector[0] = (float)done;          \
         infovector[1] = re;                   \
     }                                         \
@@ -353,6 +380,7 @@ int cilreg_dtv_dev(const float* d_input, const float* d_ref, float* d_output, fl
                    int iter, float epsil, float eta, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
                    const cilreg_opts* opts, void* stream) {
     DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    if (!d_ref) { set_error("null device pointer"); return -1; }
     if (o.precision != CILREG_FP32) { set_error("FGP-dTV keeps its fields in FP32 (dual_precision must be 0)"); return -1; }
     const int nz = dimZ <= 1 ? 1 : dimZ;
     rc = dtv_solve<float>(d_input, d_ref, d_output, lambdaPar, iter, epsil, eta, methodTV, nonneg, dimX, dimY, nz, o, s,

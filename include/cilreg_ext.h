@@ -90,7 +90,8 @@ CILREG_API int cilreg_release_cache(void);
 
 /* Device-resident solves.  d_input and d_output are device arrays of
  * dimX*dimY*dimZ floats (2D: dimZ = 1); infovector is a host float[2]
- * receiving [iterations, last residual] as in the reference.  For FGP with
+ * receiving [iterations, last residual] as in the reference (NULL skips it).
+ * A NULL d_input or d_output returns -1 before any device work.  For FGP with
  * epsil != 0, d_output's contents seed the first residual like the reference
  * C-OMP Output does. */
 CILREG_API int cilreg_rof_dev(const float *d_input, float *d_output, float *infovector, float lambdaPar, int iter, float tau,

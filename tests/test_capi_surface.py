@@ -141,3 +141,27 @@ def test_job_struct_matches_header(tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got[0] == ctypes.sizeof(CilregJob)
     assert got[1:] == [getattr(CilregJob, f).offset for f in fields]
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libcilregcuda.so not built")
+def test_null_pointers_rejected_before_device_work():
+    """Null host or device pointers return -1 with a message, before any HIP
+    call (so this runs without a GPU)."""
+    lib = ctypes.CDLL(LIB)
+    lib.cilreg_last_error.restype = ctypes.c_char_p
+    f = ctypes.c_float
+    buf = np.zeros(64, dtype=np.float32)
+    info = np.zeros(2, dtype=np.float32)
+    p = buf.ctypes.data_as(ctypes.c_void_p)
+    pi = info.ctypes.data_as(ctypes.c_void_p)
+    lib.TV_FGP_GPU_main.argtypes = [ctypes.c_void_p] * 3 + [f, ctypes.c_int, f] + [ctypes.c_int] * 6
+    for args, what in [((None, p, pi), b"input"), ((p, None, pi), b"output"), ((p, p, None), b"infovector")]:
+        assert lib.TV_FGP_GPU_main(*args, f(0.02), 5, f(0.0), 0, 0, 0, 8, 8, 1) == -1
+        assert what in lib.cilreg_last_error()
+    lib.dTV_FGP_GPU_main.argtypes = [ctypes.c_void_p] * 4 + [f, ctypes.c_int, f, f] + [ctypes.c_int] * 6
+    assert lib.dTV_FGP_GPU_main(p, None, p, pi, f(0.02), 5, f(0.0), f(0.01), 0, 0, 0, 8, 8, 1) == -1
+    assert b"reference" in lib.cilreg_last_error()
+    lib.cilreg_fgp_dev.argtypes = [ctypes.c_void_p] * 3 + [f, ctypes.c_int, f] + [ctypes.c_int] * 5 + \
+        [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.cilreg_fgp_dev(None, p, pi, f(0.02), 5, f(0.0), 0, 0, 8, 8, 1, None, None) == -1
+    assert b"null device pointer" in lib.cilreg_last_error()
