@@ -157,6 +157,35 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
             tk = tkp1;
         }
     }
+    // one iteration k on the one-iteration kernel
+    auto one = [&](int k, bool write_d, bool check, const Ctrl* c_ptr) {
+        FgpArgs<T, S> a;
+        a.f = d_f;
+        for (int c = 0; c < 3; ++c) {
+            a.P[c] = Pbuf(k, c);
+            a.Pp[c] = Pbuf(k - 1, c);
+            a.Pn[c] = Pbuf(k + 1, c);
+        }
+        a.Dout = Dbuf[k & 1];
+        a.Dprev = Dbuf[(k + 1) & 1];
+        a.lambda = lambda;
+        a.step = stepc;
+        a.mult = mult[k];
+        a.write_d = write_d;
+        a.check = check;
+        a.nonneg = nonneg != 0;
+        a.partials = partials0;
+        a.ctrl = c_ptr;
+        a.g = g;
+        if (dim == 3) {
+            if (ty == 8) launch_fgp<3, 8>(a, iso, small, s);
+            else launch_fgp<3, 4>(a, iso, small, s);
+        } else if (width == 2 && ty2d == 8) {
+            launch_fgp<2, 8>(a, iso, small, s);   // odd last iteration on the pass geometry
+        } else {
+            launch_fgp<2, 4>(a, iso, small, s);
+        }
+    };
     auto step = [&](int k, int nk, bool c0, bool c1) {
         if (nk == 2) {
             Fgp2Args<T, S> a;
@@ -174,7 +203,10 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
             a.step = stepc;
             a.mult0 = mult[k];
             a.mult1 = mult[k + 1];
-            a.write_d0 = needs_d(k) || c0;
+            // D_k of a pass's first iteration is needed in memory only as the
+            // output of a stop there (its check and iteration k+1 read it from
+            // LDS); that case recomputes it after the passes
+            a.write_d0 = 0;
             a.write_d1 = needs_d(k + 1) || c1;
             a.check0 = c0;
             a.check1 = c1;
@@ -193,37 +225,16 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
             }
             return;
         }
-        FgpArgs<T, S> a;
-        a.f = d_f;
-        for (int c = 0; c < 3; ++c) {
-            a.P[c] = Pbuf(k, c);
-            a.Pp[c] = Pbuf(k - 1, c);
-            a.Pn[c] = Pbuf(k + 1, c);
-        }
-        a.Dout = Dbuf[k & 1];
-        a.Dprev = Dbuf[(k + 1) & 1];
-        a.lambda = lambda;
-        a.step = stepc;
-        a.mult = mult[k];
-        a.write_d = needs_d(k) || c0;
-        a.check = c0;
-        a.nonneg = nonneg != 0;
-        a.partials = partials0;
-        a.ctrl = ctrl;
-        a.g = g;
-        if (dim == 3) {
-            if (ty == 8) launch_fgp<3, 8>(a, iso, small, s);
-            else launch_fgp<3, 4>(a, iso, small, s);
-        } else if (width == 2 && ty2d == 8) {
-            launch_fgp<2, 8>(a, iso, small, s);   // odd last iteration on the pass geometry
-        } else {
-            launch_fgp<2, 4>(a, iso, small, s);
-        }
+        one(k, needs_d(k) || c0, c0, ctrl);
     };
     int done = 0;
     if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials0, partials1, g.nblocks, s, step, &done, re) != 0)
         return -1;
     *iters_done = done;
+    // a stop on the first iteration of a two-iteration pass: D_done from
+    // P_done and P_{done-1}, which the no-op passes after the stop left intact
+    // (same kernel and expressions as every one-iteration pass)
+    if (done < iter && width == 2 && done % 2 == 0 && done + 1 < iter) one(done, true, false, nullptr);
     // the last D written: iteration `done` on a stop, else iter - 1
     const int last = (done < iter) ? done : iter - 1;
     CILREG_CK(hipMemcpyAsync(d_out, Dbuf[last & 1], n * sizeof(T), hipMemcpyDeviceToDevice, s));
