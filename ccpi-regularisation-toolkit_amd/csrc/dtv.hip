@@ -688,6 +688,34 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
             tk = tkp1;
         }
     }
+    // one iteration k on the one-iteration kernel
+    auto one = [&](int k, bool write_d, bool check, const Ctrl* c_ptr) {
+        DtvArgs<T> a;
+        a.f = d_f;
+        for (int c = 0; c < 3; ++c) {
+            a.B[c] = B[c];
+            a.P[c] = Pbuf(k, c);
+            a.Pp[c] = Pbuf(k - 1, c);
+            a.Pn[c] = Pbuf(k + 1, c);
+        }
+        a.Dout = Dbuf[k & 1];
+        a.Dprev = Dbuf[(k + 1) & 1];
+        a.lambda = lambda;
+        a.step = stepc;
+        a.mult = mult[k];
+        a.write_d = write_d;
+        a.check = check;
+        a.nonneg = nonneg != 0;
+        a.partials = partials;
+        a.ctrl = c_ptr;
+        a.g = g;
+        if (dim == 3) {
+            if (ty == 8) launch_dtv<3, 8>(a, iso, small, s);
+            else launch_dtv<3, 4>(a, iso, small, s);
+        } else {
+            launch_dtv<2, 4>(a, iso, small, s);
+        }
+    };
     auto step = [&](int k, int nk, bool check, bool check1) {
         if (nk == 2) {
             Dtv2Args<T> a;
@@ -706,7 +734,9 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
             a.step = stepc;
             a.mult0 = mult[k];
             a.mult1 = mult[k + 1];
-            a.write_d0 = needs_d(k) || check;
+            // D_k of a pass's first iteration stays on chip (its check and
+            // iteration k+1 read it from LDS); a stop there recomputes it
+            a.write_d0 = 0;
             a.write_d1 = needs_d(k + 1) || check1;
             a.check0 = check;
             a.check1 = check1;
@@ -719,36 +749,15 @@ int dtv_solve(const T* d_f, const T* d_ref, T* d_out, T lambda, int iter, T epsi
             else launch_dtv2<4>(a, iso, small, s);
             return;
         }
-        DtvArgs<T> a;
-        a.f = d_f;
-        for (int c = 0; c < 3; ++c) {
-            a.B[c] = B[c];
-            a.P[c] = Pbuf(k, c);
-            a.Pp[c] = Pbuf(k - 1, c);
-            a.Pn[c] = Pbuf(k + 1, c);
-        }
-        a.Dout = Dbuf[k & 1];
-        a.Dprev = Dbuf[(k + 1) & 1];
-        a.lambda = lambda;
-        a.step = stepc;
-        a.mult = mult[k];
-        a.write_d = needs_d(k) || check;
-        a.check = check;
-        a.nonneg = nonneg != 0;
-        a.partials = partials;
-        a.ctrl = ctrl;
-        a.g = g;
-        if (dim == 3) {
-            if (ty == 8) launch_dtv<3, 8>(a, iso, small, s);
-            else launch_dtv<3, 4>(a, iso, small, s);
-        } else {
-            launch_dtv<2, 4>(a, iso, small, s);
-        }
+        one(k, needs_d(k) || check, check, ctrl);
     };
     int done = 0;
     if (drive_passes(iter, width, (float)epsil, opts, ctrl, partials, partials1, g.nblocks, s, step, &done, re) != 0)
         return -1;
     *iters_done = done;
+    // a stop on the first iteration of a two-iteration pass: D_done from
+    // P_done and P_{done-1}, left intact by the no-op passes after the stop
+    if (done < iter && width == 2 && done % 2 == 0 && done + 1 < iter) one(done, true, false, nullptr);
     const int last = (done < iter) ? done : iter - 1;
     CILREG_CK(hipMemcpyAsync(d_out, Dbuf[last & 1], n * sizeof(T), hipMemcpyDeviceToDevice, s));
     CILREG_CK(hipStreamSynchronize(s));

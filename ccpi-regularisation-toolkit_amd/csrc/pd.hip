@@ -65,8 +65,10 @@ static void launch_pd2(const Pd2Args<T, S>& a, bool iso, bool small, hipStream_t
 // Device-resident PD solve.  In 3D, iterations run two per pass
 // (pd2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
 // 2D iteration, runs on the one-iteration kernel.  Each pass reads state set
-// `cur` (u_bar, P) and writes set cur ^ 1; checked iterations leave the
-// un-extrapolated U in Ubrk0 (first iteration of a pass) or Ubrk1 (second).
+// `cur` (u_bar, P) and writes set cur ^ 1.  A stop returns the un-extrapolated
+// U of the stop iteration: one-iteration passes store it on checks (Ubrk); a
+// stop inside a two-iteration pass replays that pass's iterations on the
+// one-iteration kernel afterwards.
 template <typename T, typename S>
 int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, int method, int nonneg, int nx, int ny,
              int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
@@ -105,7 +107,6 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
     oB[0] = ar.reserve(n * sizeof(T));
-    if (width == 2 && use_tol) oB[1] = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
     oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
@@ -127,6 +128,36 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     const T lt = tau / lambda;
     int cur = 0;
     std::vector<int> pass_k, pass_n, pass_out;
+    // one iteration from state set `in` to set in ^ 1; with `check` the
+    // un-extrapolated U goes to Ubrk
+    auto one = [&](int in, bool check, const Ctrl* c_ptr) {
+        const int out = in ^ 1;
+        PdArgs<T, S> a;
+        a.f = d_f;
+        a.U = (const T*)ar.at(oU[in]);
+        a.Uout = (T*)ar.at(oU[out]);
+        for (int c = 0; c < 3; ++c) {
+            const int cc = c < dim ? c : 0;
+            a.P[c] = (const S*)ar.at(oP[in][cc]);
+            a.Pout[c] = (S*)ar.at(oP[out][cc]);
+        }
+        a.Ubrk = (T*)ar.at(oB[0]);
+        a.sigma = sigma;
+        a.tau = tau;
+        a.lt = lt;
+        a.theta = 1;
+        a.nonneg = nonneg != 0;
+        a.check = check;
+        a.partials = partials0;
+        a.ctrl = c_ptr;
+        a.g = g;
+        if (dim == 3) {
+            if (ty == 8) launch_pd<3, 8>(a, iso, small, s);
+            else launch_pd<3, 4>(a, iso, small, s);
+        } else {
+            launch_pd<2, 4>(a, iso, small, s);
+        }
+    };
     auto step = [&](int k, int nk, bool c0, bool c1) {
         const int nxt = cur ^ 1;
         pass_k.push_back(k);
@@ -141,8 +172,10 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
                 a.P[c] = (const S*)ar.at(oP[cur][c]);
                 a.Pout[c] = (S*)ar.at(oP[nxt][c]);
             }
-            a.Ubrk0 = (T*)ar.at(oB[0]);
-            a.Ubrk1 = use_tol ? (T*)ar.at(oB[1]) : nullptr;
+            // the un-extrapolated U of a stop inside the pass is recomputed
+            // afterwards, so the pass stores none
+            a.Ubrk0 = nullptr;
+            a.Ubrk1 = nullptr;
             a.sigma = sigma;
             a.tau = tau;
             a.lt = lt;
@@ -157,31 +190,7 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
             if (ty == 8) launch_pd2<8>(a, iso, small, s);
             else launch_pd2<4>(a, iso, small, s);
         } else {
-            PdArgs<T, S> a;
-            a.f = d_f;
-            a.U = (const T*)ar.at(oU[cur]);
-            a.Uout = (T*)ar.at(oU[nxt]);
-            for (int c = 0; c < 3; ++c) {
-                const int cc = c < dim ? c : 0;
-                a.P[c] = (const S*)ar.at(oP[cur][cc]);
-                a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
-            }
-            a.Ubrk = (T*)ar.at(oB[0]);
-            a.sigma = sigma;
-            a.tau = tau;
-            a.lt = lt;
-            a.theta = 1;
-            a.nonneg = nonneg != 0;
-            a.check = c0;
-            a.partials = partials0;
-            a.ctrl = ctrl;
-            a.g = g;
-            if (dim == 3) {
-                if (ty == 8) launch_pd<3, 8>(a, iso, small, s);
-                else launch_pd<3, 4>(a, iso, small, s);
-            } else {
-                launch_pd<2, 4>(a, iso, small, s);
-            }
+            one(cur, c0, ctrl);
         }
         cur = nxt;
     };
@@ -194,7 +203,19 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
     if (done < iter) {
         for (size_t q = 0; q < pass_k.size(); ++q)
             if (done >= pass_k[q] && done < pass_k[q] + pass_n[q]) {
-                src = (pass_n[q] == 2 && done == pass_k[q] + 1) ? ar.at(oB[1]) : ar.at(oB[0]);
+                if (pass_n[q] == 2) {
+                    // replay the pass's iterations up to the stop one from its
+                    // input set (intact: the passes after the stop were no-ops)
+                    // with the one-iteration kernel, whose results are the same
+                    const int in = pass_out[q] ^ 1;
+                    if (done == pass_k[q]) {
+                        one(in, true, nullptr);
+                    } else {
+                        one(in, false, nullptr);
+                        one(in ^ 1, true, nullptr);
+                    }
+                }
+                src = ar.at(oB[0]);
                 break;
             }
     } else {

@@ -184,7 +184,9 @@ __global__ __launch_bounds__(TX * TY) void pd_iter_kernel(PdArgs<T, S> a) {
 // LDS planes (the last plane's mirror is the plane below, which is still in
 // the other parity buffer because the pass writes no plane beyond the
 // volume).  On a check the un-extrapolated U goes to Ubrk0 (iteration k) or
-// Ubrk1 (k+1), the output should the stopping rule fire there.
+// Ubrk1 (k+1), the output should the stopping rule fire there; a null Ubrk0 /
+// Ubrk1 skips that store (the single-volume solver recomputes the stop
+// iteration instead).
 template <typename T, typename S>
 struct Pd2Args {
     const T* f;
@@ -293,7 +295,7 @@ __device__ inline void pd2_body(const Pd2Args<T, S>& a, const TileCoord& tc) {
                         const T df = un - u0;
                         sdd0 += (double)df * (double)df;
                         suu0 += (double)un * (double)un;
-                        stnt<T>(a.Ubrk0, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], un);
+                        if (a.Ubrk0) stnt<T>(a.Ubrk0, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], un);
                     }
                     UB[zi & 1][by[s]][bx[s]] = un + theta * (un - u0);
                 }
@@ -335,7 +337,7 @@ __device__ inline void pd2_body(const Pd2Args<T, S>& a, const TileCoord& tc) {
                 const T df = un - u0;
                 sdd1 += (double)df * (double)df;
                 suu1 += (double)un * (double)un;
-                stnt<T>(a.Ubrk1, io, un);
+                if (a.Ubrk1) stnt<T>(a.Ubrk1, io, un);
             }
             stnt<T>(a.Uout, io, un + theta * (un - u0));
             stnt<T>(a.Pout[0], io, q2[0][0]);

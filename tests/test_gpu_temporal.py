@@ -264,3 +264,30 @@ def test_dtv_depth2_wide_index():
     r = _vol((12, 30, 66), 28)
     a, b = _dtv_pair(x, r, lam=0.03, iters=9, eta=0.05, wide_index=True)
     _same(a, b, "dtv wide index")
+
+
+# A stop on the first iteration of a two-iteration pass returns a value the
+# pass keeps on chip (FGP / FGP-dTV: D_k; PD: the un-extrapolated U_k), and a
+# stop on the second one PD's U_{k+1}: the solvers recompute those after the
+# passes.  Both positions must occur and match one iteration per pass.
+@pytest.mark.parametrize("solver", ["fgp", "pd", "dtv"])
+def test_depth2_stop_on_either_iteration_of_a_pass(solver):
+    from ccpi.filters import device as D
+    x = _vol((16, 48, 80), 41)
+    r = _vol((16, 48, 80), 42)
+    parities = set()
+    for ce in (1, 3):
+        for tol in (3e-2, 1e-2, 3e-3, 1e-3, 3e-4):
+            if solver == "fgp":
+                kw = dict(lam=0.03, iters=200, tol=tol, check_every=ce)
+                a, b = D.fgp(x, depth=1, **kw), D.fgp(x, depth=2, **kw)
+            elif solver == "pd":
+                kw = dict(lam=0.03, iters=200, tol=tol, L=12.0, check_every=ce)
+                a, b = D.pd(x, depth=1, **kw), D.pd(x, depth=2, **kw)
+            else:
+                kw = dict(lam=0.03, iters=200, tol=tol, eta=0.05, check_every=ce)
+                a, b = D.dtv(x, r, depth=1, **kw), D.dtv(x, r, depth=2, **kw)
+            _same(a, b, f"{solver} tol={tol} check_every={ce}")
+            if int(a[1][0]) < 200:
+                parities.add(int(a[1][0]) % 2)
+    assert parities == {0, 1}, f"stop positions exercised: {parities}"
