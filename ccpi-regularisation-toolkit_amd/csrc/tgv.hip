@@ -59,9 +59,9 @@ static void launch_tgv2(const Tgv2Args<T, S>& a, bool small, hipStream_t s) {
 // Device-resident TGV solve.  In 3D, iterations run two per pass
 // (tgv2_iter_kernel) unless opts.depth == 1; an odd last iteration, and every
 // 2D iteration, runs on the one-iteration kernel.  Each pass reads state set
-// `cur` and writes set cur ^ 1; a pass whose first iteration is checked also
-// writes u_bar after that iteration to Ualt, the output should the stopping
-// rule fire there.
+// `cur` and writes set cur ^ 1.  The output of a stop on a pass's first
+// iteration (u_bar after it) stays on chip; that case replays the iteration on
+// the one-iteration kernel from the pass's input set afterwards.
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
@@ -93,15 +93,13 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
 
     Arena ar;
-    size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oAlt = 0, oCtrl, oPart;
+    size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
     for (int b = 0; b < 2; ++b) {
         oU[b] = ar.reserve(n * sizeof(T));
         for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
         for (int c = 0; c < nq; ++c) oQ[b][c] = ar.reserve(n * sizeof(S));
     }
-    const bool use_tol = epsil != (T)0;
-    if (width == 2 && use_tol) oAlt = ar.reserve(n * sizeof(T));
     oCtrl = ar.reserve(sizeof(Ctrl));
     oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
@@ -129,6 +127,41 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
 
     int cur = 0;
     std::vector<int> pass_k, pass_n, pass_out;   // where each pass left its state
+    // one iteration from state set `in` to set in ^ 1
+    auto one = [&](int in, bool check, const Ctrl* c_ptr) {
+        const int out = in ^ 1;
+        TgvArgs<T, S> a;
+        a.f = d_f;
+        a.U = (const T*)ar.at(oU[in]);
+        a.Uout = (T*)ar.at(oU[out]);
+        for (int c = 0; c < 3; ++c) {
+            const int cc = c < dim ? c : 0;
+            a.V[c] = (const S*)ar.at(oV[in][cc]);
+            a.Vout[c] = (S*)ar.at(oV[out][cc]);
+            a.P[c] = (const S*)ar.at(oP[in][cc]);
+            a.Pout[c] = (S*)ar.at(oP[out][cc]);
+        }
+        for (int c = 0; c < 6; ++c) {
+            const int cc = c < nq ? c : 0;
+            a.Q[c] = (const S*)ar.at(oQ[in][cc]);
+            a.Qout[c] = (S*)ar.at(oQ[out][cc]);
+        }
+        a.lambda = lambda;
+        a.tau = tau;
+        a.sigma = tau;
+        a.alpha1 = alpha1;
+        a.alpha0 = alpha0;
+        a.check = check;
+        a.partials = partials0;
+        a.ctrl = c_ptr;
+        a.g = g;
+        if (dim == 3) {
+            if (ty == 8) launch_tgv<3, 8>(a, small, s);
+            else launch_tgv<3, 4>(a, small, s);
+        } else {
+            launch_tgv<2, 4>(a, small, s);
+        }
+    };
     auto step = [&](int k, int nk, bool c0, bool c1) {
         const int nxt = cur ^ 1;
         pass_k.push_back(k);
@@ -149,7 +182,7 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
                 a.Q[c] = (const S*)ar.at(oQ[cur][c]);
                 a.Qout[c] = (S*)ar.at(oQ[nxt][c]);
             }
-            a.Ualt = use_tol ? (T*)ar.at(oAlt) : nullptr;
+            a.Ualt = nullptr;   // a stop on the first iteration is replayed afterwards
             a.lambda = lambda;
             a.tau = tau;
             a.sigma = tau;
@@ -164,37 +197,7 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
             if (ty == 8) launch_tgv2<8>(a, small, s);
             else launch_tgv2<4>(a, small, s);
         } else {
-            TgvArgs<T, S> a;
-            a.f = d_f;
-            a.U = (const T*)ar.at(oU[cur]);
-            a.Uout = (T*)ar.at(oU[nxt]);
-            for (int c = 0; c < 3; ++c) {
-                const int cc = c < dim ? c : 0;
-                a.V[c] = (const S*)ar.at(oV[cur][cc]);
-                a.Vout[c] = (S*)ar.at(oV[nxt][cc]);
-                a.P[c] = (const S*)ar.at(oP[cur][cc]);
-                a.Pout[c] = (S*)ar.at(oP[nxt][cc]);
-            }
-            for (int c = 0; c < 6; ++c) {
-                const int cc = c < nq ? c : 0;
-                a.Q[c] = (const S*)ar.at(oQ[cur][cc]);
-                a.Qout[c] = (S*)ar.at(oQ[nxt][cc]);
-            }
-            a.lambda = lambda;
-            a.tau = tau;
-            a.sigma = tau;
-            a.alpha1 = alpha1;
-            a.alpha0 = alpha0;
-            a.check = c0;
-            a.partials = partials0;
-            a.ctrl = ctrl;
-            a.g = g;
-            if (dim == 3) {
-                if (ty == 8) launch_tgv<3, 8>(a, small, s);
-                else launch_tgv<3, 4>(a, small, s);
-            } else {
-                launch_tgv<2, 4>(a, small, s);
-            }
+            one(cur, c0, ctrl);
         }
         cur = nxt;
     };
@@ -207,7 +210,13 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     const void* src = nullptr;
     for (size_t q = 0; q < pass_k.size(); ++q) {
         if (last >= pass_k[q] && last < pass_k[q] + pass_n[q]) {
-            src = (pass_n[q] == 2 && last == pass_k[q]) ? ar.at(oAlt) : ar.at(oU[pass_out[q]]);
+            if (pass_n[q] == 2 && last == pass_k[q]) {
+                // a stop on the pass's first iteration: replay it from the
+                // pass's input set (intact: the passes after the stop were
+                // no-ops) into its output set
+                one(pass_out[q] ^ 1, false, nullptr);
+            }
+            src = ar.at(oU[pass_out[q]]);
             break;
         }
     }

@@ -331,7 +331,7 @@ struct Tgv2Args {
     S* Vout[3];
     S* Pout[3];
     S* Qout[6];
-    T* Ualt;            // u_bar after iteration k (written when check0: the output of a stop at k)
+    T* Ualt;            // u_bar after iteration k (written when check0 and non-null: the output of a stop at k)
     T lambda, tau, sigma, alpha1, alpha0;
     int check0, check1;
     double* partials0;
@@ -464,7 +464,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                             const T df = ubn - cu[0];
                             sdd0 += (double)df * (double)df;
                             suu0 += (double)ubn * (double)ubn;
-                            stnt<T>(a.Ualt, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], ubn);
+                            if (a.Ualt) stnt<T>(a.Ualt, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], ubn);
                         }
                     }
                     UV[zi & 1][0][by[s]][bx[s]] = ubn;
