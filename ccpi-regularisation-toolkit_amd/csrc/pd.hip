@@ -100,9 +100,8 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
         g = plan_geom(small ? pd_kernel_ptr<3, 4, T, S, uint32_t>(iso) : pd_kernel_ptr<3, 4, T, S, uint64_t>(iso),
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
 
-    const bool use_tol = epsil != (T)0;
     Arena ar;
-    size_t oU[2], oP[2][3], oB[2] = {0, 0}, oCtrl, oPart;
+    size_t oU[2], oP[2][3], oB[1] = {0}, oCtrl, oPart;
     for (int b = 0; b < 2; ++b) oU[b] = ar.reserve(n * sizeof(T));
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
