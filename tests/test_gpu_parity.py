@@ -5,7 +5,7 @@ library) runs through ``ccpi.filters.regularisers`` with ``device='gpu'``, i.e.
 the drop-in entry points TV_ROF_GPU_main ... TGV_GPU_main of libcilregcuda.so,
 and again through the device-resident extension API.
 
-Tolerances: the north-star bar is 1e-5 relative L2 for FP32 runs.  The
+Tolerances: the required bar is 1e-5 relative L2 for FP32 runs.  The
 kernels restate the reference's arithmetic operation for operation, so a
 fixed-iteration run (tol = 0) is also required to be bit-exact.  With a
 tolerance the stopping decision depends on the residual norm, which the GPU
