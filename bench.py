@@ -576,7 +576,7 @@ def main():
                                                                 "ndf3d", "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(volumes and requests shard across GPUs without communication)",
               file=sys.stderr)
         sys.exit(2)

@@ -186,7 +186,7 @@ Geom plan_geom(K kernel, int threads, int nx, int ny, int nz, int ty, int zc_hin
 // check1) launches the pass that advances iterations k .. k+nk-1 (nk is 1 or,
 // when the caller's pass width is 2 and two iterations remain, 2); check0 /
 // check1 say whether iteration k / k+1 evaluates the stopping rule, and the
-// pass leaves those residual sums in partials0 / partials1.  The driver then
+// pass leaves those residual sums in partials0 / partials1.  drive_passes then
 // launches the finalisers in iteration order, polls the stop flag every
 // opts.poll iterations without draining the queue, and on return has
 // synchronised the stream and filled *iters_done and *re.  With opts.profile
