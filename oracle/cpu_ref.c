@@ -513,8 +513,13 @@ REAL FN(cr_pd)(const REAL *Input, REAL *U, REAL *infovector, REAL lambdaPar, int
             for (j = 0; j < dimY; j++)
                 for (i = 0; i < dimX; i++) {
                     idx_t c = sxy * k + (idx_t)dimX * j + i;
-                    P1[c] += sigma * ((i == dimX - 1) ? U[c - 1] - U[c] : U[c + 1] - U[c]);
-                    P2[c] += sigma * ((j == dimY - 1) ? U[c - dimX] - U[c] : U[c + dimX] - U[c]);
+                    /* the mirrored neighbour of the last index; on an axis of
+                     * extent 1 (where the reference reads out of bounds) the
+                     * point itself */
+                    idx_t cx = (i == dimX - 1) ? (dimX > 1 ? c - 1 : c) : c + 1;
+                    idx_t cy = (j == dimY - 1) ? (dimY > 1 ? c - dimX : c) : c + dimX;
+                    P1[c] += sigma * (U[cx] - U[c]);
+                    P2[c] += sigma * (U[cy] - U[c]);
                     if (ndim == 3)
                         P3[c] += sigma * ((k == nz - 1) ? U[c - sxy] - U[c] : U[c + sxy] - U[c]);
                 }
@@ -567,11 +572,13 @@ static void sb_sweep(REAL *U, const REAL *f, const REAL *Up, REAL *const Dd[3], 
 #pragma omp parallel for
     for (k = 0; k < nz; k++) {
         idx_t j, i;
-        idx_t kp = (k + 1 == nz) ? k - 1 : k + 1, km = (k - 1 < 0) ? k + 1 : k - 1;
+        /* mirror: +1, or -1 at the last index (-1: +1 at index 0); on an axis
+         * of extent 1, where the reference reads out of bounds, the point itself */
+        idx_t kp = mir_p(k, nz), km = mir_m(k, nz);
         for (j = 0; j < ny; j++) {
-            idx_t jp = (j + 1 == ny) ? j - 1 : j + 1, jm = (j - 1 < 0) ? j + 1 : j - 1;
+            idx_t jp = mir_p(j, ny), jm = mir_m(j, ny);
             for (i = 0; i < nx; i++) {
-                idx_t ip = (i + 1 == nx) ? i - 1 : i + 1, im = (i - 1 < 0) ? i + 1 : i - 1;
+                idx_t ip = mir_p(i, nx), im = mir_m(i, nx);
                 idx_t c = sxy * k + nx * j + i;
                 idx_t cim = sxy * k + nx * j + im, cjm = sxy * k + nx * jm + i;
                 REAL acc;
@@ -606,11 +613,11 @@ static void sb_shrink_bregman(const REAL *U, REAL *const Dd[3], REAL *const Bb[3
 #pragma omp parallel for
     for (k = 0; k < nz; k++) {
         idx_t j, i;
-        idx_t kp = (k + 1 == nz) ? k - 1 : k + 1;
+        idx_t kp = mir_p(k, nz);   /* extent 1: the point itself (see sb_sweep) */
         for (j = 0; j < ny; j++) {
-            idx_t jp = (j + 1 == ny) ? j - 1 : j + 1;
+            idx_t jp = mir_p(j, ny);
             for (i = 0; i < nx; i++) {
-                idx_t ip = (i + 1 == nx) ? i - 1 : i + 1;
+                idx_t ip = mir_p(i, nx);
                 idx_t c = sxy * k + nx * j + i;
                 REAL g[3], v[3] = {0, 0, 0};
                 int d;

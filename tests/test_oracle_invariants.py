@@ -33,6 +33,11 @@ reference fixtures.
   floors): with lambda_LLT = 0 it is ROF-TV with the floor 1e-12 instead of
   1e-8, within 1e-6 of the ROF oracle; scaling f and both lambdas by 2
   scales the output by 2 within 1e-6.
+* An axis of extent 1 (where the reference's mirrored neighbour reads out of
+  bounds) equals, bit for bit, the same axis at extent 2 with the data
+  duplicated, where the reference's mirror is defined and lands on the
+  duplicate: every difference along the axis is zero either way (ROF, PD,
+  SB; checked against the reference library on the extent-2 input too).
 * The FP64 build agrees with the FP32 build to FP32 accuracy.
 """
 import numpy as np
@@ -250,3 +255,29 @@ def test_llt_scale_equivariance(shape):
     err = np.abs(b.astype(np.float64) - 2 * a).max()
     assert err < 1e-6, err
     assert np.abs(a - f).max() > 1e-3
+
+
+EXT1_CASES = [
+    ("rof", {"lambda": 0.04, "iters": 20, "tau": 0.002, "tol": 0.0}),
+    ("pd", {"lambda": 0.04, "iters": 20, "tol": 0.0, "L": 12.0, "method": 0, "nonneg": 1}),
+    ("pd", {"lambda": 0.04, "iters": 20, "tol": 0.0, "L": 12.0, "method": 1, "nonneg": 0}),
+    ("sb", {"mu": 0.05, "iters": 10, "tol": 0.0, "method": 0}),
+    ("sb", {"mu": 0.05, "iters": 10, "tol": 0.0, "method": 1}),
+]
+
+
+@pytest.mark.parametrize("shape,axis", [((1, 70), 0), ((65, 1), 1), ((9, 1, 66), 1), ((7, 40, 1), 2)],
+                         ids=["2d-y1", "2d-x1", "3d-y1", "3d-x1"])
+@pytest.mark.parametrize("method,params", EXT1_CASES, ids=[f"{m}{i}" for i, (m, _) in enumerate(EXT1_CASES)])
+def test_extent1_axis_equals_duplicated_extent2(shape, axis, method, params):
+    x = (np.random.default_rng(11).random(shape) * 2 - 0.5).astype(np.float32)
+    x2 = np.ascontiguousarray(np.concatenate([x, x], axis=axis))
+    y1, i1 = oracle.cpu_ref().run(method, x, params)
+    y2, i2 = oracle.cpu_ref().run(method, x2, params)
+    assert np.isfinite(y1).all()
+    for half in np.split(y2, 2, axis=axis):
+        assert bits_equal(np.ascontiguousarray(half), y1)
+    ref = oracle.reference()
+    if ref is not None:
+        y2r, _ = ref.run(method, x2, params)
+        assert bits_equal(y2r, y2)
