@@ -18,7 +18,8 @@ Also reported:
   also on bounded samples of every other measured workload
   (cpu_baseline.other_workloads);
 * secondary: TGV 3D 192x512x512 (500 iterations; FP32, BF16 and FP16 dual
-  storage against an FP64 solve), PD-TV 3D 256^3 (400 iterations, L = 12),
+  storage against an FP64 solve), PD-TV 3D 256^3 (400 iterations, L = 12; FP32,
+  BF16 and FP16 storage of P against an FP64 solve),
   ROF-TV 2D 512^2 (2000 iterations); 64 uneven volumes (half FGP, half ROF)
   solved one after another vs in one batched hipGraph launch (FP32 and BF16
   duals); a Poisson request stream (200 req/s, ROF/FGP/PD/TGV, 2D and 3D)
@@ -227,6 +228,26 @@ def secondary(D, torch):
         "voxel_iters_per_s": n * info[0] / dt, "iterations": int(info[0]), "seconds": round(dt, 4),
         "kernel_ms_mean": m, "alg_bytes_per_voxel_iter": 36,
         "kernel_alg_GBps": 36 * n / (m * 1e-3) / 1e9 if m else None}
+    # PD precision study: P stored in FP32 / BF16 / FP16 against an FP64 solve (400 iterations, no tolerance)
+    log("bench: PD-TV 3D 256^3 precision study")
+    ref, _ = D.pd_f64(x.double(), 0.02, 400, 0.0, 12.0, 0, 1)
+    ref_norm = float(torch.linalg.vector_norm(ref))
+    for prec in ("fp32", "bf16", "fp16"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y, info = D.pd(x, 0.02, 400, 0.0, 12.0, 0, 1, precision=prec)
+        torch.cuda.synchronize()
+        dtp = time.perf_counter() - t0
+        D.pd(x, 0.02, 50, 0.0, 12.0, 0, 1, precision=prec, profile=True)
+        mp, _, _ = D.last_profile()
+        b = 36 if prec == "fp32" else 24   # P at 2 B: read u_bar, f (8) + P (6); write u_bar (4) + P (6)
+        out[f"pd_tv_3d_256_{prec}_duals"] = {
+            "voxel_iters_per_s": n * info[0] / dtp, "iterations": int(info[0]), "seconds": round(dtp, 4),
+            "kernel_ms_mean": mp, "alg_bytes_per_voxel_iter": b,
+            "kernel_alg_GBps": b * n / (mp * 1e-3) / 1e9 if mp else None,
+            "rel_l2_vs_fp64": float(torch.linalg.vector_norm(y.double() - ref)) / ref_norm}
+        del y
+    del ref
     # SB 3D (two kernels per iteration)
     log("bench: SB-TV 3D 256^3")
     D.sb(x, 0.02, 10, 0.0, 0)

@@ -99,6 +99,7 @@ _SIGS = {
     "cilreg_d4_dev": [_vp, _vp, _vp, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _vp],
     "cilreg_dtv_dev": [_vp, _vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _i, _i, _i, _i, _i, _vp, _vp],
+    "cilreg_pd_dev_f64": [_vp, _vp, _vp, _d, _i, _d, _d, _i, _i, _i, _i, _i, _vp, _vp],
     "cilreg_tgv_dev_f64": [_vp, _vp, _vp, _d, _d, _d, _i, _d, _d, _i, _i, _i, _vp, _vp],
     "cilreg_fgp_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
     "cilreg_pd_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _i, _vp, _vp],
@@ -218,6 +219,12 @@ def fgp_f64(x, lam, iters, tol=0.0, method=0, nonneg=0, out=None, **kw):
     import torch
     out = _prep(x, out, torch.float64)
     return _run("cilreg_fgp_dev_f64", x, out, torch.float64, [lam, iters, tol, method, nonneg], make_opts(**kw))
+
+
+def pd_f64(x, lam, iters, tol=0.0, L=12.0, method=0, nonneg=0, out=None, **kw):
+    import torch
+    out = _prep(x, out, torch.float64)
+    return _run("cilreg_pd_dev_f64", x, out, torch.float64, [lam, iters, tol, L, method, nonneg], make_opts(**kw))
 
 
 def tgv_f64(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):

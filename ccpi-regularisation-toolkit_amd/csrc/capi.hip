@@ -398,6 +398,16 @@ int cilreg_fgp_dev_f64(const double* d_input, double* d_output, double* infovect
     DEV_EPILOGUE()
 }
 
+int cilreg_pd_dev_f64(const double* d_input, double* d_output, double* infovector, double lambdaPar, int iter,
+                      double epsil, double lipschitz_const, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
+                      const cilreg_opts* opts, void* stream) {
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    rc = pd_solve<double, double>(d_input, d_output, lambdaPar, iter, epsil, lipschitz_const, methodTV, nonneg, dimX,
+                                  dimY, nz, o, s, &done, &re);
+    DEV_EPILOGUE()
+}
+
 int cilreg_tgv_dev_f64(const double* d_input, double* d_output, double* infovector, double lambda, double alpha1,
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts* opts, void* stream) {

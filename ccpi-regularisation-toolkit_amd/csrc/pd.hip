@@ -243,5 +243,7 @@ template int pd_solve<float, bf16_t>(const float*, float*, float, int, float, fl
                                      const Opts&, hipStream_t, int*, float*, const WarmState*);
 template int pd_solve<float, _Float16>(const float*, float*, float, int, float, float, int, int, int, int, int,
                                        const Opts&, hipStream_t, int*, float*, const WarmState*);
+template int pd_solve<double, double>(const double*, double*, double, int, double, double, int, int, int, int, int,
+                                      const Opts&, hipStream_t, int*, float*, const WarmState*);
 
 }  // namespace cilreg

@@ -126,6 +126,9 @@ CILREG_API int cilreg_dtv_dev(const float *d_input, const float *d_ref, float *d
 CILREG_API int cilreg_fgp_dev_f64(const double *d_input, double *d_output, double *infovector, double lambdaPar, int iter,
                        double epsil, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
                        const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_pd_dev_f64(const double *d_input, double *d_output, double *infovector, double lambdaPar, int iter,
+                      double epsil, double lipschitz_const, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
+                      const cilreg_opts *opts, void *stream);
 CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, double *infovector, double lambda, double alpha1,
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts *opts, void *stream);

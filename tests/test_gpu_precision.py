@@ -1,6 +1,6 @@
 """FP64 solves and reduced-precision dual storage on the GPU.
 
-* The FP64 instantiations (cilreg_fgp_dev_f64, cilreg_tgv_dev_f64) are the
+* The FP64 instantiations (cilreg_{fgp,pd,tgv}_dev_f64) are the
   oracle of the precision study; they must equal the FP64 build of the CPU
   restatement bit for bit (same operation order, IEEE double division and
   sqrt, no contraction).
@@ -31,6 +31,11 @@ def test_fp64_gpu_matches_fp64_oracle(inp):
     a, ia = D.fgp_f64(xt, 0.03, 60, 0.0, 0, 1)
     b, ib = oracle.cpu_ref("f64").fgp(x, 0.03, 60, 0.0, 0, 1)
     assert np.array_equal(a.cpu().numpy(), b)
+    for method, nonneg in ((0, 1), (1, 0)):
+        for iters in (60, 61):   # two-iteration passes, and one ending on a one-iteration tail
+            a, ia = D.pd_f64(xt, 0.03, iters, 0.0, 12.0, method, nonneg)
+            b, ib = oracle.cpu_ref("f64").pd(x, 0.03, iters, 0.0, 12.0, method, nonneg)
+            assert np.array_equal(a.cpu().numpy(), b), (method, nonneg, iters)
 
 
 def _phantom():
@@ -63,6 +68,19 @@ def test_fgp_reduced_precision_duals(prec):
     y, info = D.fgp(torch.from_numpy(x).cuda(), 0.02, 200, 0.0, 0, 1, precision=prec)
     err = rel_l2(y.cpu().numpy(), ref.cpu().numpy())
     print(f"FGP 40x96x112, 200 iterations, {prec} duals: rel L2 vs FP64 = {err:.3e}")
+    assert err <= BOUND[prec], err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_pd_reduced_precision_duals(prec):
+    import torch
+    from ccpi.filters import device as D
+    x = _phantom()
+    ref, _ = D.pd_f64(torch.from_numpy(x.astype(np.float64)).cuda(), 0.02, 200, 0.0, 12.0, 0, 1)
+    y, info = D.pd(torch.from_numpy(x).cuda(), 0.02, 200, 0.0, 12.0, 0, 1, precision=prec)
+    err = rel_l2(y.cpu().numpy(), ref.cpu().numpy())
+    print(f"PD 40x96x112, 200 iterations, {prec} duals: rel L2 vs FP64 = {err:.3e}")
     assert err <= BOUND[prec], err
 
 
