@@ -306,6 +306,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         }
     }
     T dbz[2] = {}, fz[2] = {};     // db and f of plane zi - 1
+    T bq1[3] = {}, bq2[3] = {};    // B of the own point at planes zi - 1, zi - 2 (for S3)
     double sdd = 0.0, suu = 0.0;
     for (int zi = zstart; zi <= z1 + 1; ++zi) {
         // ---- S1a: own values of plane zi into LDS; U0 of the plane above
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
             if (DIM == 3) gr[2] = U2p[sb_p(zq, nz) & 1][by[0]][bx[0]] - u2;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                bv[d] = ldx<T>(a.B[d], i);
+                bv[d] = bq2[d];   // loaded at S1a two planes ago
                 v[d] = gr[d] + bv[d];
             }
             if (ANISO) {
@@ -428,6 +429,11 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
                 sdd += (double)df * (double)df;
                 suu += (double)u2 * (double)u2;
             }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            bq2[d] = bq1[d];
+            bq1[d] = s1 ? bv[0][d] : (T)0;
         }
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
