@@ -40,6 +40,7 @@ struct JobHdr {
     int start_level;    // level at which the job runs its iteration 0
     int iters;          // iteration count
     int check_every;    // residual cadence, 0 = no checks
+    int tiles;          // tiles per workgroup (1 in 3D; 2D: job_tiles_per_block)
     Ctrl* ctrl;
     double* partials;
 };
@@ -120,6 +121,31 @@ __device__ inline int group_prologue(const J* jobs, int njobs, int total_blocks,
     return k;
 }
 
+// A 2D tile is one plane of 64 x TY points, too little work to amortise the
+// job lookup of group_prologue: a large 2D job runs up to four consecutive
+// tiles per workgroup (3D tiles march a z chunk and run one each).  The host
+// chooses the count per job (JobHdr::tiles) and sizes the job's share of the
+// grid accordingly.
+
+// Runs body(tile) for the workgroup's tiles local * R .. local * R + R - 1 that
+// exist (R = tiles, uniform over the workgroup).  LDS is reused, so tiles
+// after the first start behind a barrier.  3D instantiations keep the single
+// tile without a loop.
+template <int DIM, int TY, class Body>
+__device__ inline void for_tiles(const Geom& g, const JobHdr& h, int local, Body&& body) {
+    if constexpr (DIM == 3) {
+        body(tile_of_logical(g, TY, local));
+    } else {
+        const int tiles = __builtin_amdgcn_readfirstlane(h.tiles);
+        for (int r = 0; r < tiles; ++r) {
+            const int tl = local * tiles + r;
+            if (tl >= g.nblocks) break;
+            if (r) __syncthreads();
+            body(tile_of_logical(g, TY, tl));
+        }
+    }
+}
+
 // A pointer read from a job descriptor, made explicitly uniform: it then
 // lives in scalar registers like a kernel argument instead of a vector
 // register pair per pointer (the group kernels would otherwise need about 30
@@ -163,7 +189,7 @@ __global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* job
     a.partials = uni(j->h.partials);
     a.ctrl = uni(j->h.ctrl);
     a.g = uni_geom(j->h.g);
-    rof_body<DIM, TY, T, I>(a, tile_of_logical(a.g, TY, local));
+    for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { rof_body<DIM, TY, T, I>(a, tc); });
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
@@ -195,7 +221,7 @@ __global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* 
     a.partials = uni(j->h.partials);
     a.ctrl = uni(j->h.ctrl);
     a.g = uni_geom(j->h.g);
-    fgp_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
+    for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { fgp_body<DIM, TY, T, S, ISO, I>(a, tc); });
 }
 
 // FGP groups advance two levels per launch: a job at an even iteration
@@ -210,7 +236,6 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
     if (k < 0) return;
     const int ce = j->h.check_every, iters = j->h.iters;
     auto needs_d = [&](int q) { return q == iters - 1 || (ce > 0 && (q % ce == 0 || (q + 1) % ce == 0)); };
-    const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
     if (k + 1 < iters) {
         Fgp2Args<T, S> a;
         #pragma unroll
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
         a.partials1 = uni(j->partials1);
         a.ctrl = uni(j->h.ctrl);
         a.g = uni_geom(j->h.g);
-        fgp2_body<TY, T, S, ISO, I, DIM>(a, tc);
+        for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { fgp2_body<TY, T, S, ISO, I, DIM>(a, tc); });
     } else {
         FgpArgs<T, S> a;
         #pragma unroll
@@ -258,7 +283,7 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
         a.partials = uni(j->h.partials);
         a.ctrl = uni(j->h.ctrl);
         a.g = uni_geom(j->h.g);
-        fgp_body<DIM, TY, T, S, ISO, I>(a, tc);
+        for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { fgp_body<DIM, TY, T, S, ISO, I>(a, tc); });
     }
 }
 
@@ -288,7 +313,7 @@ __global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jo
     a.partials = uni(j->h.partials);
     a.ctrl = uni(j->h.ctrl);
     a.g = uni_geom(j->h.g);
-    pd_body<DIM, TY, T, S, ISO, I>(a, tile_of_logical(a.g, TY, local));
+    for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { pd_body<DIM, TY, T, S, ISO, I>(a, tc); });
 }
 
 template <int DIM, int TY, typename T, typename S, typename I>
@@ -324,7 +349,7 @@ __global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* 
     a.partials = uni(j->h.partials);
     a.ctrl = uni(j->h.ctrl);
     a.g = uni_geom(j->h.g);
-    tgv_body<DIM, TY, T, S, I>(a, tile_of_logical(a.g, TY, local));
+    for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { tgv_body<DIM, TY, T, S, I>(a, tc); });
 }
 
 // 3D PD groups with two-iteration passes: pass q = k / 2 of a job reads
@@ -907,6 +932,7 @@ private:
         d.h.start_level = j.start_level;
         d.h.iters = j.pub.iter;
         d.h.check_every = j.pub.epsil != 0.0f ? opts_.check_every : 0;
+        d.h.tiles = job_tiles(j);
         d.h.ctrl = j.ctrl;
         d.h.partials = j.partials;
     }
@@ -995,6 +1021,19 @@ private:
         }
     }
 
+    // tiles per workgroup: a 2D job with at least 4096 tiles runs 2, from
+    // 8192 on 4 (enough workgroups remain to fill the device on their own)
+    static int job_tiles(const EJob& j) {
+        if (j.dim != 2) return 1;
+        const int r = j.g.nblocks / 2048;
+        return r < 1 ? 1 : (r > 4 ? 4 : r);
+    }
+    // workgroups of a job in its group's grid
+    static int job_blocks(const EJob& j) {
+        const int r = job_tiles(j);
+        return (j.g.nblocks + r - 1) / r;
+    }
+
     int build_tables() {
         for (auto& kv : tables_) { kv.second.jobs.clear(); kv.second.host.clear(); kv.second.njobs = 0; }
         for (EJob* j : active_) tables_[key_of(*j)].jobs.push_back(j);
@@ -1013,7 +1052,7 @@ private:
                 describe(*j, off, t.host);
                 t.offsets.push_back(off);
                 t.ends.push_back(j->start_level + j->pub.iter);
-                off += j->g.nblocks;
+                off += job_blocks(*j);
                 if (j->pub.epsil != 0.0f)
                     fins.push_back(FinJob{j->ctrl, j->partials, j->partials + 2 * (size_t)j->g.nblocks, j->g.nblocks,
                                           j->start_level, j->pub.iter,
