@@ -674,7 +674,11 @@ public:
         levels += levels & 1;   // even, so jobs are admitted at even levels (two-level FGP passes)
         if (admit() != 0) return -1;
         if (!active_.empty() && levels > 0) {
-            if (build_tables() != 0) return -1;
+            // the job tables depend only on the set of active jobs
+            if (tables_dirty_) {
+                if (build_tables() != 0) return -1;
+                tables_dirty_ = false;
+            }
             if (launch_levels(level_, level_ + levels, false) != 0) return -1;
             level_ += levels;
             CILREG_CK(hipStreamSynchronize(s_));
@@ -765,6 +769,7 @@ private:
     Opts opts_;
     hipStream_t s_;
     int level_ = 0;
+    bool tables_dirty_ = true;   // the active set changed since the tables were built
     long long next_ticket_ = 0;
     std::map<long long, std::unique_ptr<EJob>> all_;
     std::deque<EJob*> pending_;
@@ -916,6 +921,7 @@ private:
                 continue;
             }
             active_.push_back(j);
+            tables_dirty_ = true;
         }
         return 0;
     }
@@ -1206,10 +1212,16 @@ private:
     }
 
     int retire() {
+        // only jobs with a tolerance can stop early: the others finish at
+        // start_level + iter, and their control blocks stay as reset
         std::vector<Ctrl> c(active_.size());
-        for (size_t i = 0; i < active_.size(); ++i)
+        bool copied = false;
+        for (size_t i = 0; i < active_.size(); ++i) {
+            if (active_[i]->pub.epsil == 0.0f) continue;
             CILREG_CK(hipMemcpyAsync(&c[i], active_[i]->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s_));
-        CILREG_CK(hipStreamSynchronize(s_));
+            copied = true;
+        }
+        if (copied) CILREG_CK(hipStreamSynchronize(s_));
         std::vector<EJob*> still;
         for (size_t i = 0; i < active_.size(); ++i) {
             EJob& j = *active_[i];
@@ -1245,6 +1257,7 @@ private:
             j.info[1] = c[i].re;
             j.state = J_DONE;
         }
+        if (still.size() != active_.size()) tables_dirty_ = true;
         active_.swap(still);
         return 0;
     }
