@@ -199,6 +199,52 @@ __global__ __launch_bounds__(256) void finalize_kernel(const double* partials, i
     finalize_body(partials, nblocks, ctrl, epsil, iter);
 }
 
+// The stopping rule for iterations `iter` and `iter` + 1 of a two-iteration
+// pass in one launch: the same fixed-order sums as finalize_body over each
+// partials array, then the rule for `iter` and, unless that stopped the solve,
+// for `iter` + 1 -- the state two finalize_kernel launches in a row leave.
+__global__ __launch_bounds__(256) void finalize2_kernel(const double* p0, const double* p1, int nblocks, Ctrl* ctrl,
+                                                        float epsil, int iter) {
+    __shared__ double red[4 * 256];
+    if (ctrl->stopped) return;
+    double a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        a0 += p0[2 * i];
+        b0 += p0[2 * i + 1];
+        a1 += p1[2 * i];
+        b1 += p1[2 * i + 1];
+    }
+    red[threadIdx.x] = a0;
+    red[256 + threadIdx.x] = b0;
+    red[512 + threadIdx.x] = a1;
+    red[768 + threadIdx.x] = b1;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) red[256 * q + threadIdx.x] += red[256 * q + threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int count = ctrl->count;
+        for (int h = 0; h < 2; ++h) {
+            const double sd = red[512 * h], su = red[512 * h + 256];
+            const float re = (float)(sqrt(sd) / sqrt(su));
+            ctrl->re = re;
+            ctrl->sum_d = sd;
+            ctrl->sum_u = su;
+            if (re < epsil) count++;
+            ctrl->count = count;
+            if (count > 3) {
+                ctrl->stop_iter = iter + h;
+                ctrl->stopped = 1;
+                break;
+            }
+        }
+    }
+}
+
 __global__ void ctrl_reset_kernel(Ctrl* ctrl) {
     if (threadIdx.x == 0) {
         ctrl->stopped = 0;
@@ -229,6 +275,11 @@ __global__ void tail_bytes_kernel(unsigned char* d, const unsigned char* s, size
 
 void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s) {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, partials, nblocks, ctrl, epsil, iter);
+}
+
+void launch_finalize2(const double* p0, const double* p1, int nblocks, Ctrl* ctrl, float epsil, int iter,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(finalize2_kernel, dim3(1), dim3(256), 0, s, p0, p1, nblocks, ctrl, epsil, iter);
 }
 
 void launch_ctrl_reset(Ctrl* ctrl, hipStream_t s) {

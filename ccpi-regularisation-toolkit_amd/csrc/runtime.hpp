@@ -111,6 +111,10 @@ __device__ inline void finalize_body(const double* partials, int nblocks, Ctrl* 
 }
 
 void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s);
+// Both iterations of a two-iteration pass in one launch (iter and iter + 1),
+// with the result of two launch_finalize calls in a row.
+void launch_finalize2(const double* p0, const double* p1, int nblocks, Ctrl* ctrl, float epsil, int iter,
+                      hipStream_t s);
 // Zero the control block (a small fill kernel, safe under graph capture).
 void launch_ctrl_reset(Ctrl* ctrl, hipStream_t s);
 // Fill a float/double/16-bit array with zeros, or copy one array to another.
@@ -227,8 +231,12 @@ int drive_passes(int iter, int width, float epsil, const Opts& opts, Ctrl* d_ctr
             pass_n.push_back(nk);
         }
         ++npass;
-        if (c0) launch_finalize(d_partials0, nblocks, d_ctrl, epsil, k, s);
-        if (c1) launch_finalize(d_partials1, nblocks, d_ctrl, epsil, k + 1, s);
+        if (c0 && c1) {
+            launch_finalize2(d_partials0, d_partials1, nblocks, d_ctrl, epsil, k, s);
+        } else {
+            if (c0) launch_finalize(d_partials0, nblocks, d_ctrl, epsil, k, s);
+            if (c1) launch_finalize(d_partials1, nblocks, d_ctrl, epsil, k + 1, s);
+        }
         k += nk;
         since_poll += nk;
         if (use_tol && since_poll >= opts.poll && k < iter) {
