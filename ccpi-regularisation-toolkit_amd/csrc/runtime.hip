@@ -206,7 +206,9 @@ __global__ __launch_bounds__(256) void finalize_kernel(const double* partials, i
 __global__ __launch_bounds__(256) void finalize2_kernel(const double* p0, const double* p1, int nblocks, Ctrl* ctrl,
                                                         float epsil, int iter) {
     __shared__ double red[4 * 256];
-    if (ctrl->stopped) return;
+    // the flag load overlaps the partials' loads (after a stop they hold
+    // stale sums, which are then discarded)
+    const int stopped = ctrl->stopped;
     double a0 = 0, b0 = 0, a1 = 0, b1 = 0;
     for (int i = threadIdx.x; i < nblocks; i += 256) {
         a0 += p0[2 * i];
@@ -214,6 +216,7 @@ __global__ __launch_bounds__(256) void finalize2_kernel(const double* p0, const 
         a1 += p1[2 * i];
         b1 += p1[2 * i + 1];
     }
+    if (stopped) return;
     red[threadIdx.x] = a0;
     red[256 + threadIdx.x] = b0;
     red[512 + threadIdx.x] = a1;
