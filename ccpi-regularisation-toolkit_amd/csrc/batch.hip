@@ -483,13 +483,19 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
-// `second`: the level is the second iteration of a two-iteration pass, whose
-// residual sums are in partials1.
-__global__ __launch_bounds__(256) void group_finalize_kernel(const FinJob* jobs, int level, int second) {
+// The stopping rule of every job that checks at `level` (partials) or, in a
+// two-iteration pass (`width` 2), at `level` + 1 (partials1): one workgroup
+// per job, both checks of a pass in one launch, with the decisions of two
+// single-level finalisers in a row.
+__global__ __launch_bounds__(256) void group_finalize_kernel(const FinJob* jobs, int level, int width) {
     const FinJob& j = jobs[blockIdx.x];
+    if (j.check_every <= 0) return;
     const int k = level - j.start_level;
-    if (k < 0 || k >= j.iters || j.check_every <= 0 || k % j.check_every != 0) return;
-    finalize_body(second ? j.partials1 : j.partials, j.nblocks, j.ctrl, j.epsil, k);
+    const bool c0 = k >= 0 && k < j.iters && k % j.check_every == 0;
+    const bool c1 = width == 2 && k + 1 >= 0 && k + 1 < j.iters && (k + 1) % j.check_every == 0;
+    if (c0 && c1) finalize2_body(j.partials, j.partials1, j.nblocks, j.ctrl, j.epsil, k);
+    else if (c0) finalize_body(j.partials, j.nblocks, j.ctrl, j.epsil, k);
+    else if (c1) finalize_body(j.partials1, j.nblocks, j.ctrl, j.epsil, k + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1197,9 +1203,7 @@ private:
                 else launch_group<2, 4>(k, t, nact, total, lv, st);
                 if (t.fin_n) {
                     const FinJob* fj = (const FinJob*)fin_dev_ + t.fin_off;
-                    hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st, fj, lv, 0);
-                    if (width == 2)
-                        hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st, fj, lv + 1, 1);
+                    hipLaunchKernelGGL(group_finalize_kernel, dim3(t.fin_n), dim3(256), 0, st, fj, lv, width);
                 }
             }
             CILREG_CK(hipEventRecord(events_[gi], st));

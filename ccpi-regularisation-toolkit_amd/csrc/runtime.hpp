@@ -110,6 +110,55 @@ __device__ inline void finalize_body(const double* partials, int nblocks, Ctrl* 
     }
 }
 
+// The stopping rule for iterations `iter` and `iter` + 1 of a two-iteration
+// pass in one workgroup: the same fixed-order sums as finalize_body over each
+// partials array, then the rule for `iter` and, unless that stopped the solve,
+// for `iter` + 1 -- the state two finalize_body calls in a row leave.
+__device__ inline void finalize2_body(const double* p0, const double* p1, int nblocks, Ctrl* ctrl, float epsil,
+                                      int iter) {
+    __shared__ double red[4 * 256];
+    // the flag load overlaps the partials' loads (after a stop they hold
+    // stale sums, which are then discarded)
+    const int stopped = ctrl->stopped;
+    double a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        a0 += p0[2 * i];
+        b0 += p0[2 * i + 1];
+        a1 += p1[2 * i];
+        b1 += p1[2 * i + 1];
+    }
+    if (stopped) return;
+    red[threadIdx.x] = a0;
+    red[256 + threadIdx.x] = b0;
+    red[512 + threadIdx.x] = a1;
+    red[768 + threadIdx.x] = b1;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) red[256 * q + threadIdx.x] += red[256 * q + threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int count = ctrl->count;
+        for (int h = 0; h < 2; ++h) {
+            const double sd = red[512 * h], su = red[512 * h + 256];
+            const float re = (float)(sqrt(sd) / sqrt(su));
+            ctrl->re = re;
+            ctrl->sum_d = sd;
+            ctrl->sum_u = su;
+            if (re < epsil) count++;
+            ctrl->count = count;
+            if (count > 3) {
+                ctrl->stop_iter = iter + h;
+                ctrl->stopped = 1;
+                break;
+            }
+        }
+    }
+}
+
 void launch_finalize(const double* partials, int nblocks, Ctrl* ctrl, float epsil, int iter, hipStream_t s);
 // Both iterations of a two-iteration pass in one launch (iter and iter + 1),
 // with the result of two launch_finalize calls in a row.
