@@ -256,12 +256,31 @@ __device__ inline void llt_body(const DiffArgs<float>& a, const TileCoord& tc) {
         R1s[ly + 1][lx + 1] = r1;
         R2s[ly + 1][lx + 1] = r2;
         if (r.lane) {
-            float h1, h2, h3, k1, k2;
-            llt_point<DIM, I>(a, r.x, r.y, z, h1, h2, h3, k1, k2);
-            L1s[r.sy][r.sx] = k1;
-            L2s[r.sy][r.sx] = k2;
-            R1s[r.sy][r.sx] = h1;
-            R2s[r.sy][r.sx] = h2;
+            // only what the tile reads at the lane's slot: L1 on the columns
+            // and D2 on the left one, L2 on the rows and D1 on the upper one
+            const I zo = (I)z * sxy, rc = zo + (I)r.y * nx + (I)r.x;
+            const float uc = ldx<float>(a.U, rc);
+            const float uip = ldx<float>(a.U, zo + (I)r.y * nx + (I)mirror_p(r.x, g.nx));
+            const float uim = ldx<float>(a.U, zo + (I)r.y * nx + (I)mirror_m(r.x, g.nx));
+            const float ujp = ldx<float>(a.U, zo + (I)mirror_p(r.y, g.ny) * nx + (I)r.x);
+            const float ujm = ldx<float>(a.U, zo + (I)mirror_m(r.y, g.ny) * nx + (I)r.x);
+            float mk = 0.f;
+            if (DIM == 3) {
+                const float ukp = ldx<float>(a.U, (I)mirror_p(z, g.nz) * sxy + (I)r.y * nx + (I)r.x);
+                const float ukm = ldx<float>(a.U, (I)mirror_m(z, g.nz) * sxy + (I)r.y * nx + (I)r.x);
+                mk = Mixed<float>::minmod_sq(ukp - uc, uc - ukm);
+            }
+            if (r.sx == 0 || r.sx == TX + 1) {
+                L1s[r.sy][r.sx] = llt_ratio(uip - 2.0f * uc + uim);
+                if (r.sx == 0)
+                    R2s[r.sy][r.sx] = rof_d2<DIM, float>(uip - uc, Mixed<float>::minmod_sq(ujp - uc, uc - ujm), mk,
+                                                          1.0e-12);
+            } else {
+                L2s[r.sy][r.sx] = llt_ratio(ujp - 2.0f * uc + ujm);
+                if (r.sy == 0)
+                    R1s[r.sy][r.sx] = rof_d1<DIM, float>(ujp - uc, Mixed<float>::minmod_sq(uip - uc, uc - uim), mk,
+                                                          1.0e-12);
+            }
         }
         __syncthreads();
         if (own) {

@@ -23,10 +23,26 @@ struct RofArgs {
 __device__ inline int mirror_p(int i, int n) { return i + 1 >= n ? (i - 1 >= 0 ? i - 1 : 0) : i + 1; }
 __device__ inline int mirror_m(int i, int n) { return i - 1 < 0 ? (i + 1 < n ? i + 1 : 0) : i - 1; }
 
-// The fluxes D1 (along y), D2 (along x) and, in 3D, D3 (along z) at a point
-// from u there (uc) and at its mirrored neighbours (ROF_TV_core.c:102-266).
-// eps is the floor added in double under the square root: 1e-8 for ROF-TV,
-// 1e-12 for the ROF part of LLT-ROF (LLT_ROF_core.c:22,198-363).
+// The flux D1 (along y) from the forward difference fj and the squared
+// minmods across x (mi) and, in 3D, z (mk): ROF_TV_core.c:131-137 (3D, double
+// sqrt), :157-161 (2D).  eps is the floor added in double under the square
+// root: 1e-8 for ROF-TV, 1e-12 for the ROF part of LLT-ROF
+// (LLT_ROF_core.c:22,198-363).
+template <int DIM, typename T>
+__device__ inline T rof_d1(T fj, T mi, T mk, double eps) {
+    if (DIM == 3) return fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk, eps);
+    return fj / Mixed<T>::rof_sqrt_f(fj * fj + mi, eps);
+}
+
+// The flux D2 (along x): ROF_TV_core.c:194-200 (3D), :220-224 (2D).
+template <int DIM, typename T>
+__device__ inline T rof_d2(T fi, T mj, T mk, double eps) {
+    if (DIM == 3) return fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk, eps);
+    return fi / Mixed<T>::rof_sqrt_f(fi * fi + mj, eps);
+}
+
+// The fluxes D1, D2 and, in 3D, D3 (along z, ROF_TV_core.c:257-263) at a
+// point from u there (uc) and at its mirrored neighbours.
 template <int DIM, typename T>
 __device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T ukm, T& d1, T& d2, T& d3,
                                      double eps = 1.0e-8) {
@@ -35,14 +51,12 @@ __device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T 
     if (DIM == 3) {
         const T fk = ukp - uc, bk = uc - ukm;
         const T mk = Mixed<T>::minmod_sq(fk, bk);
-        // ROF_TV_core.c:131-137 (double sqrt), :194-200, :257-263
-        d1 = fj / Mixed<T>::rof_sqrt_d(fj * fj + mi + mk, eps);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj + mk, eps);
+        d1 = rof_d1<DIM, T>(fj, mi, mk, eps);
+        d2 = rof_d2<DIM, T>(fi, mj, mk, eps);
         d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi, eps);
     } else {
-        // ROF_TV_core.c:157-161, :220-224
-        d1 = fj / Mixed<T>::rof_sqrt_f(fj * fj + mi, eps);
-        d2 = fi / Mixed<T>::rof_sqrt_f(fi * fi + mj, eps);
+        d1 = rof_d1<DIM, T>(fj, mi, T(0), eps);
+        d2 = rof_d2<DIM, T>(fi, mj, T(0), eps);
         d3 = 0;
     }
 }
@@ -150,11 +164,14 @@ __device__ inline void rof_body(const RofArgs<T>& a, const TileCoord& tc) {
             const T uxm = Us[sy][rxm >= 0 && rxm < TX + 2 ? rxm : sx];
             const T uyp = Us[ryp >= 0 && ryp < TY + 2 ? ryp : sy][sx];
             const T uym = Us[rym >= 0 && rym < TY + 2 ? rym : sy][sx];
-            T h1, h2, h3;
-            rof_flux_math<DIM, T>(hc, uxp, uxm, uyp, uym, hp, hm, h1, h2, h3);
-            if (rin) {
-                if (d2lane) D2l[sy - 1][0] = h2;
-                else D1l[0][sx - 1] = h1;
+            // only the flux the lane stores: D2 on the column, D1 on the row
+            const T mk = DIM == 3 ? Mixed<T>::minmod_sq(hp - hc, hc - hm) : T(0);
+            if (d2lane) {
+                const T h2 = rof_d2<DIM, T>(uxp - hc, Mixed<T>::minmod_sq(uyp - hc, hc - uym), mk, 1.0e-8);
+                if (rin) D2l[sy - 1][0] = h2;
+            } else {
+                const T h1 = rof_d1<DIM, T>(uyp - hc, Mixed<T>::minmod_sq(uxp - hc, hc - uxm), mk, 1.0e-8);
+                if (rin) D1l[0][sx - 1] = h1;
             }
         }
         __syncthreads();
