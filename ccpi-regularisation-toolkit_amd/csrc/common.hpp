@@ -251,6 +251,29 @@ template <> struct Mixed<float> {
         const float m = (0.5f * (float)(sa + sb)) * (fa < fb ? fa : fb);
         return m * m;
     }
+    // N squared minmods at once.  For finite a, b the reference's product is
+    // +-min(|a|,|b|) when the signs agree and 0 otherwise (a factor 0.5
+    // arises only when one of them is 0, which makes the minimum 0): the
+    // median of (a, b, 0).  It differs only when a or b is NaN or when a and
+    // b are infinities of opposite sign; a*b is then NaN or -inf, and a wave
+    // holding any lane whose a*b is not above -inf (overflow to -inf
+    // included) takes the reference's expression instead.
+    template <int N>
+    __device__ static inline void minmod_sq_n(const float (&a)[N], const float (&b)[N], float (&m)[N]) {
+        bool odd = false;
+        #pragma unroll
+        for (int q = 0; q < N; ++q) odd = odd || !(a[q] * b[q] > -__builtin_inff());
+        if (__builtin_amdgcn_ballot_w64(odd)) {
+            #pragma unroll
+            for (int q = 0; q < N; ++q) m[q] = minmod_sq(a[q], b[q]);
+        } else {
+            #pragma unroll
+            for (int q = 0; q < N; ++q) {
+                const float v = __builtin_amdgcn_fmed3f(a[q], b[q], 0.f);
+                m[q] = v * v;
+            }
+        }
+    }
 };
 template <> struct Mixed<double> {
     __device__ static inline double rof_sqrt_f(double s, double eps = 1.0e-8) { return sqrt(s + eps); }
@@ -261,6 +284,11 @@ template <> struct Mixed<double> {
         double fa = fabs(a), fb = fabs(b);
         double m = (0.5 * (double)(sa + sb)) * (fa < fb ? fa : fb);
         return m * m;
+    }
+    template <int N>
+    __device__ static inline void minmod_sq_n(const double (&a)[N], const double (&b)[N], double (&m)[N]) {
+        #pragma unroll
+        for (int q = 0; q < N; ++q) m[q] = minmod_sq(a[q], b[q]);
     }
 };
 

@@ -47,16 +47,19 @@ template <int DIM, typename T>
 __device__ inline void rof_flux_math(T uc, T uip, T uim, T ujp, T ujm, T ukp, T ukm, T& d1, T& d2, T& d3,
                                      double eps = 1.0e-8) {
     const T fi = uip - uc, bi = uc - uim, fj = ujp - uc, bj = uc - ujm;
-    const T mi = Mixed<T>::minmod_sq(fi, bi), mj = Mixed<T>::minmod_sq(fj, bj);
     if (DIM == 3) {
         const T fk = ukp - uc, bk = uc - ukm;
-        const T mk = Mixed<T>::minmod_sq(fk, bk);
+        T m[3];
+        Mixed<T>::minmod_sq_n({fi, fj, fk}, {bi, bj, bk}, m);
+        const T mi = m[0], mj = m[1], mk = m[2];
         d1 = rof_d1<DIM, T>(fj, mi, mk, eps);
         d2 = rof_d2<DIM, T>(fi, mj, mk, eps);
         d3 = fk / Mixed<T>::rof_sqrt_f(fk * fk + mj + mi, eps);
     } else {
-        d1 = rof_d1<DIM, T>(fj, mi, T(0), eps);
-        d2 = rof_d2<DIM, T>(fi, mj, T(0), eps);
+        T m[2];
+        Mixed<T>::minmod_sq_n({fi, fj}, {bi, bj}, m);
+        d1 = rof_d1<DIM, T>(fj, m[0], T(0), eps);
+        d2 = rof_d2<DIM, T>(fi, m[1], T(0), eps);
         d3 = 0;
     }
 }
@@ -165,12 +168,16 @@ __device__ inline void rof_body(const RofArgs<T>& a, const TileCoord& tc) {
             const T uyp = Us[ryp >= 0 && ryp < TY + 2 ? ryp : sy][sx];
             const T uym = Us[rym >= 0 && rym < TY + 2 ? rym : sy][sx];
             // only the flux the lane stores: D2 on the column, D1 on the row
-            const T mk = DIM == 3 ? Mixed<T>::minmod_sq(hp - hc, hc - hm) : T(0);
+            // (the across-difference pair: y for D2, x for D1)
+            const T fa = d2lane ? uyp - hc : uxp - hc, ba = d2lane ? hc - uym : hc - uxm;
+            T m[2];
+            Mixed<T>::minmod_sq_n({fa, DIM == 3 ? hp - hc : T(0)}, {ba, DIM == 3 ? hc - hm : T(0)}, m);
+            const T mk = DIM == 3 ? m[1] : T(0);
             if (d2lane) {
-                const T h2 = rof_d2<DIM, T>(uxp - hc, Mixed<T>::minmod_sq(uyp - hc, hc - uym), mk, 1.0e-8);
+                const T h2 = rof_d2<DIM, T>(uxp - hc, m[0], mk, 1.0e-8);
                 if (rin) D2l[sy - 1][0] = h2;
             } else {
-                const T h1 = rof_d1<DIM, T>(uyp - hc, Mixed<T>::minmod_sq(uxp - hc, hc - uxm), mk, 1.0e-8);
+                const T h1 = rof_d1<DIM, T>(uyp - hc, m[0], mk, 1.0e-8);
                 if (rin) D1l[0][sx - 1] = h1;
             }
         }
