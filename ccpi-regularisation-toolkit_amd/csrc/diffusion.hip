@@ -100,31 +100,31 @@ __device__ inline void store_partials(const DiffArgs<float>& a, const TileCoord&
 // NDF
 // ---------------------------------------------------------------------------
 // The penalty's flux for a neighbour difference d (Diffusion_core.c:166-231);
-// powf(x, 2) is x * x.
-__device__ inline float ndf_flux(float d, float sigma, int penalty) {
+// powf(x, 2) is x * x.  PEN is the penalty (0 linear), a template argument so
+// each kernel holds one straight-line flux.
+template <int PEN>
+__device__ inline float ndf_flux(float d, float sigma) {
     const float ad = fabsf(d);
     const float sg = (float)((d > 0.f) - (d < 0.f));
-    switch (penalty) {
-        case 1: return ad > sigma ? sg : d / sigma;
-        case 2: {
-            const float q = d / sigma;
-            return d / (1.0f + q * q);
-        }
-        case 3: {
-            if (ad <= sigma) {
-                const float q = d / sigma;
-                const float t = 1.0f - q * q;
-                return d * (t * t);
-            }
-            return 0.0f;
-        }
-        case 4: return ad > sigma ? 0.0f : d;
-        case 5: return ad <= 2.0f * sigma ? (ad > sigma ? sg : d / sigma) : 0.0f;
-        default: return d;   // linear diffusion
+    if constexpr (PEN == 1) return ad > sigma ? sg : d / sigma;
+    if constexpr (PEN == 2) {
+        const float q = d / sigma;
+        return d / (1.0f + q * q);
     }
+    if constexpr (PEN == 3) {
+        if (ad <= sigma) {
+            const float q = d / sigma;
+            const float t = 1.0f - q * q;
+            return d * (t * t);
+        }
+        return 0.0f;
+    }
+    if constexpr (PEN == 4) return ad > sigma ? 0.0f : d;
+    if constexpr (PEN == 5) return ad <= 2.0f * sigma ? (ad > sigma ? sg : d / sigma) : 0.0f;
+    return d;   // linear diffusion
 }
 
-template <int DIM, int TY, typename I>
+template <int DIM, int TY, typename I, int PEN>
 __device__ inline void ndf_body(const DiffArgs<float>& a, const TileCoord& tc) {
     const Geom& g = a.g;
     constexpr int NT = TX * TY;
@@ -161,9 +161,9 @@ __device__ inline void ndf_body(const DiffArgs<float>& a, const TileCoord& tc) {
                 d[4] = up - uc;            // u
                 d[5] = um - uc;            // d
             }
-            if (a.penalty != 0) {
+            if constexpr (PEN != 0) {
 #pragma unroll
-                for (int q = 0; q < (DIM == 3 ? 6 : 4); ++q) d[q] = ndf_flux(d[q], a.sigma, a.penalty);
+                for (int q = 0; q < (DIM == 3 ? 6 : 4); ++q) d[q] = ndf_flux<PEN>(d[q], a.sigma);
             }
             float s = d[0] + d[1] + d[2] + d[3];
             if (DIM == 3) s = s + d[4] + d[5];
@@ -178,10 +178,10 @@ __device__ inline void ndf_body(const DiffArgs<float>& a, const TileCoord& tc) {
     if (a.check) store_partials<TY>(a, tc, tid, sdd, suu);
 }
 
-template <int DIM, int TY, typename I>
+template <int DIM, int TY, typename I, int PEN>
 __global__ __launch_bounds__(TX * TY) void ndf_iter_kernel(DiffArgs<float> a) {
     if (solve_stopped(a.ctrl)) return;
-    ndf_body<DIM, TY, I>(a, tile_of_block(a.g, TY));
+    ndf_body<DIM, TY, I, PEN>(a, tile_of_block(a.g, TY));
 }
 
 // ---------------------------------------------------------------------------
@@ -416,10 +416,18 @@ __global__ __launch_bounds__(TX * TY) void d4_iter_kernel(DiffArgs<float> a) {
 enum Method { M_LLT, M_NDF, M_D4 };
 
 template <int DIM, int TY, typename I>
-const void* kernel_of(Method m) {
+const void* kernel_of(Method m, int penalty) {
     switch (m) {
         case M_LLT: return (const void*)llt_iter_kernel<DIM, TY, I>;
-        case M_NDF: return (const void*)ndf_iter_kernel<DIM, TY, I>;
+        case M_NDF:
+            switch (penalty) {
+                case 1: return (const void*)ndf_iter_kernel<DIM, TY, I, 1>;
+                case 2: return (const void*)ndf_iter_kernel<DIM, TY, I, 2>;
+                case 3: return (const void*)ndf_iter_kernel<DIM, TY, I, 3>;
+                case 4: return (const void*)ndf_iter_kernel<DIM, TY, I, 4>;
+                case 5: return (const void*)ndf_iter_kernel<DIM, TY, I, 5>;
+                default: return (const void*)ndf_iter_kernel<DIM, TY, I, 0>;
+            }
         default: return (const void*)d4_iter_kernel<DIM, TY, I>;
     }
 }
@@ -427,7 +435,7 @@ const void* kernel_of(Method m) {
 template <int DIM, int TY>
 void launch(Method m, const DiffArgs<float>& a, bool small, hipStream_t s) {
     dim3 block(TX, TY), grid(a.g.nblocks);
-    const void* k = small ? kernel_of<DIM, TY, uint32_t>(m) : kernel_of<DIM, TY, uint64_t>(m);
+    const void* k = small ? kernel_of<DIM, TY, uint32_t>(m, a.penalty) : kernel_of<DIM, TY, uint64_t>(m, a.penalty);
     DiffArgs<float> arg = a;
     void* args[] = {&arg};
     (void)hipLaunchKernel(k, grid, block, args, 0, s);
@@ -453,11 +461,11 @@ int diffusion_solve(Method m, const float* d_f, float* d_out, const DiffArgs<flo
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
     else if (ty == 8)
-        g = plan_geom(small ? kernel_of<3, 8, uint32_t>(m) : kernel_of<3, 8, uint64_t>(m), TX * 8, nx, ny, nz, 8,
-                      opts.zchunk);
+        g = plan_geom(small ? kernel_of<3, 8, uint32_t>(m, proto.penalty) : kernel_of<3, 8, uint64_t>(m, proto.penalty),
+                      TX * 8, nx, ny, nz, 8, opts.zchunk);
     else
-        g = plan_geom(small ? kernel_of<3, 4, uint32_t>(m) : kernel_of<3, 4, uint64_t>(m), TX * 4, nx, ny, nz, 4,
-                      opts.zchunk);
+        g = plan_geom(small ? kernel_of<3, 4, uint32_t>(m, proto.penalty) : kernel_of<3, 4, uint64_t>(m, proto.penalty),
+                      TX * 4, nx, ny, nz, 4, opts.zchunk);
     Arena ar;
     size_t oU[2], oCtrl, oPart;
     oU[0] = ar.reserve(n * sizeof(float));
