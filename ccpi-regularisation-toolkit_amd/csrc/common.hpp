@@ -239,7 +239,23 @@ template <> struct Mixed<float> {
     // (float)sqrt((double)s + eps): the 3D D1, ROF_TV_core.c:136, LLT_ROF_core.c:232
     __device__ static inline float rof_sqrt_d(float s, double eps = 1.0e-8) { return (float)sqrt((double)s + eps); }
     // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
-    __device__ static inline float pd_div(float num, float lt) { return (float)((double)num / (1.0 + (double)lt)); }
+    //
+    // Without the double division: q = num * RN(1/c), c = 1 + lt, is within
+    // 2^-52 |num/c| of the quotient and d = 2^-50 |q| (exact) exceeds that
+    // bound plus half an ulp, so RN_d(num/c) lies in [RN_d(q-d), RN_d(q+d)];
+    // rounding to float is monotone, so when both ends round to the same
+    // float the double quotient does too.  A wave holding any lane where
+    // they differ (a float rounding boundary within 2^-50 |q| of q, about one
+    // lane in 2^26, or a non-finite num) divides instead.
+    __device__ static inline float pd_div(float num, float lt) {
+        const double c = 1.0 + (double)lt;
+        const double q = (double)num * (1.0 / c);
+        const double d = fabs(q) * 0x1p-50;
+        const float lo = (float)(q - d), hi = (float)(q + d);
+        float r = lo;
+        if (__builtin_amdgcn_ballot_w64(!(lo == hi))) r = (float)((double)num / c);
+        return r;
+    }
     // squared minmod, ROF_TV_core.c:132-133.  The reference multiplies in
     // double (fabs promotes) and rounds the product to float; the factor is
     // 0, +-0.5 or +-1, so the double product is exact and its rounding to
