@@ -206,8 +206,14 @@ struct Ctrl {
     double sum_d, sum_u;  // last check's sums (diagnostics)
 };
 
-// Sum (a, b) over a 256-thread block in a fixed order; result valid in thread 0.
-__device__ inline void block_sum2(double& a, double& b, double* lds /* >= 8 doubles */) {
+// LDS scratch of block_sum2: two doubles per wave of the largest workgroup
+// (1024 threads).
+constexpr int kRedSlots = 2 * (1024 / 64);
+
+// Sum (a, b) over the workgroup in a fixed order (within each wave, then over
+// the waves in index order); result valid in thread 0.  lds holds two doubles
+// per wave of the workgroup (kRedSlots covers every launch shape).
+__device__ inline void block_sum2(double& a, double& b, double* lds) {
     #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);

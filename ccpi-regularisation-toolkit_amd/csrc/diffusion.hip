@@ -88,7 +88,7 @@ __device__ inline void add_resid(float un, float u, double& sdd, double& suu) {
 
 template <int TY>
 __device__ inline void store_partials(const DiffArgs<float>& a, const TileCoord& tc, int tid, double sdd, double suu) {
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
     block_sum2(sdd, suu, red);
     if (tid == 0) {
         a.partials[2 * tc.lin] = sdd;

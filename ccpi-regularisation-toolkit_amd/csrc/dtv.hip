@@ -109,7 +109,7 @@ __device__ inline void dtv_body(const DtvArgs<T>& a, const TileCoord& tc) {
     __shared__ T Rs1[TY + 2][TX + 2];   // R'_1 on [x0-1, x0+64] x [y0-1, y0+TY]
     __shared__ T Rs2[TY + 2][TX + 2];   // R'_2 likewise
     __shared__ T Ds[2][TY + 1][TX + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T lam = a.lambda, mult = a.mult;
@@ -362,7 +362,7 @@ __device__ inline void dtv2_body(const Dtv2Args<T>& a, const TileCoord& tc) {
     __shared__ T Rk2[BH + 1][BW + 1];
     __shared__ T R1s[BH + 1][BW + 1];      // R'_{k+1},1 and ,2 of the current plane
     __shared__ T R2s[BH + 1][BW + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T lam = a.lambda, m0 = a.mult0, m1 = a.mult1, step = a.step;

@@ -45,7 +45,7 @@ __device__ inline void fgp_body(const FgpArgs<T, S>& a, const TileCoord& tc) {
     const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T Ds[2][TY + 1][TX + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T lam = a.lambda, mult = a.mult;
@@ -292,7 +292,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     __shared__ T Rk2[BH + 1][BW + 1];
     __shared__ T R1s[BH + 1][BW + 1];      // R_{k+1},1 and ,2 of the current plane
     __shared__ T R2s[BH + 1][BW + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T lam = a.lambda, m0 = a.mult0, m1 = a.mult1, step = a.step;

@@ -93,7 +93,7 @@ __device__ inline void pd_body(const PdArgs<T, S>& a, const TileCoord& tc) {
     const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T Pl[2][TY + 1][TX + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const T sigma = a.sigma;
@@ -244,7 +244,7 @@ __device__ inline void pd2_body(const Pd2Args<T, S>& a, const TileCoord& tc) {
 
     __shared__ T Q[2][BH + 1][BW + 1];     // P1, P2 read at x-1 / y-1
     __shared__ T UB[2][BH + 1][BW + 1];    // u_bar of k+1, planes by parity
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;

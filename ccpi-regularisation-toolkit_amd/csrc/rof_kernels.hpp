@@ -116,7 +116,7 @@ __device__ inline void rof_body(const RofArgs<T>& a, const TileCoord& tc) {
     __shared__ T Us[TY + 2][TX + 2];    // U of plane z: tile and ring
     __shared__ T D1l[TY + 1][TX];       // [ly + 1][lx]; row 0 = y0 - 1
     __shared__ T D2l[TY][TX + 1];       // [ly][lx + 1]; column 0 = x0 - 1
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
 
     // mirrored neighbour slots (loop-invariant) of the own point; threads

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(TX * TY) void sb_update_kernel(SbArgs<T> a) {
     const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T Ul[2][TY + 1][TX + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
 
     T below = 0;   // U2 at the own point, plane z-1
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
     __shared__ T U2p[2][BH][BW];   // U2 planes by z mod 2
     __shared__ T U0s[BH][BW];      // U0, D_1, B_1, D_2, B_2 of plane zi (x / y neighbours of S1)
     __shared__ T D0s[BH][BW], B0s[BH][BW], D1s[BH][BW], B1s[BH][BW];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;
 

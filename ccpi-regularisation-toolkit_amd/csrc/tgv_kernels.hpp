@@ -191,7 +191,7 @@ __device__ inline void tgv_body(const TgvArgs<T, S>& a, const TileCoord& tc) {
     const int hcx = hown ? hx : 0, hcy = hown ? hy : 0;
 
     __shared__ T L[L_N][TY + 1][TX + 1];
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
 
     // channels of plane z-1 at the own point: P3, Q33, Q13, Q23
@@ -388,7 +388,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
 
     __shared__ T PQ[L_N][BH + 1][BW + 1];      // P, Q channels read at x-1 / y-1
     __shared__ T UV[2][4][BH + 1][BW + 1];     // u_bar, v_bar of k+1, planes by parity
-    __shared__ double red[8];
+    __shared__ double red[kRedSlots];
 
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const int nz = g.nz, z0 = tc.z0, z1 = tc.z1;

@@ -325,3 +325,55 @@ def test_rof_family_nonfinite_inputs_match_oracle(shape, method, params):
     assert nan_ref.any() and not nan_ref.all()
     assert np.array_equal(nan_ref, nan_gpu)
     assert bits_equal(y[~nan_gpu], y_ref[~nan_ref])
+
+
+RESID_CASES = [
+    ("rof", {"lambda": 0.04, "tau": 0.002}),
+    ("fgp", {"lambda": 0.04, "method": 0, "nonneg": 1}),
+    ("pd", {"lambda": 0.04, "L": 12.0, "method": 0, "nonneg": 1}),
+    ("sb", {"mu": 0.05, "method": 0}),
+    ("tgv", {"lambda": 0.04, "alpha1": 1.0, "alpha0": 2.0, "L2": 12.0}),
+    ("dtv", {"lambda": 0.04, "eta": 0.05, "method": 0, "nonneg": 1}),
+    ("llt", {"lambda_rof": 0.04, "lambda_llt": 0.01, "tau": 0.002}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.1, "tau": 0.02, "penalty": 1}),
+    ("d4", {"lambda": 1.0, "sigma": 0.05, "tau": 1e-3}),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("iters", [6, 7])
+@pytest.mark.parametrize("method,params", RESID_CASES, ids=[m for m, _ in RESID_CASES])
+def test_reported_residual_is_the_fp64_residual(method, params, iters):
+    """The residual a solve reports after checking every iteration equals
+    |u_N - u_(N-1)| / |u_N| computed in FP64 from the iterates of fixed-count
+    runs (PD, SB: the oracle's residual), on a 3D volume whose 512-thread
+    workgroups sum eight waves' partials, so every wave's contribution
+    reaches the stopping rule."""
+    import torch
+    from ccpi.filters import device as D
+    rng = np.random.default_rng(21)
+    x = torch.from_numpy(rng.random((20, 67, 70)).astype(np.float32)).cuda()
+    p = dict(params)
+    if method == "dtv":
+        p["ref"] = torch.from_numpy(rng.random((20, 67, 70)).astype(np.float32)).cuda()
+    if method in ("pd", "sb"):
+        # their rules compare iterates other than consecutive outputs (PD's
+        # primal before over-relaxation, PD_TV_core.c:81-101; SB's output
+        # after its first Gauss-Seidel sweep, SB_TV_core.c:63-87): against the oracle's
+        # residual at the reference's check cadence (every 5), summed
+        # serially in FP32 as the reference does
+        import oracle
+        pr = dict(p, iters=iters, tol=1e-30)
+        _, info_ref = oracle.cpu_ref().run(method, x.cpu().numpy(), pr)
+        _, info = D.run(method, x, pr)
+        assert int(info[0]) == int(info_ref[0]) == iters
+        assert abs(float(info[1]) - float(info_ref[1])) <= 1e-4 * float(info_ref[1])
+        return
+    u1, _ = D.run(method, x, dict(p, iters=iters - 1, tol=0.0))
+    u2, _ = D.run(method, x, dict(p, iters=iters, tol=0.0))
+    _, info = D.run(method, x, dict(p, iters=iters, tol=1e-30), check_every=1)
+    a = u2.double().cpu().numpy()
+    b = u1.double().cpu().numpy()
+    re = np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert int(info[0]) == iters
+    assert abs(float(info[1]) - re) <= 1e-5 * re, (float(info[1]), re)
