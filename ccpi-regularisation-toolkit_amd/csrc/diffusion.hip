@@ -178,10 +178,104 @@ __device__ inline void ndf_body(const DiffArgs<float>& a, const TileCoord& tc) {
     if (a.check) store_partials<TY>(a, tc, tid, sdd, suu);
 }
 
+// g(dw) from F = g(D), the flux a neighbour computed for its own difference
+// D across the shared edge (penalty PEN >= 1, sigma > 0).  dw = -D exactly
+// unless both are +0 (a - b = -(b - a) in round-to-nearest, except equal
+// operands, which give +0 both ways), and every penalty is odd except where
+// it returns the literal 0: there g(dw) = 0, on zero g(dw) = dw, elsewhere
+// g(dw) = -F, bit for bit (NaN operands included: the literal-zero tests are
+// written as the penalty's own).
+template <int PEN>
+__device__ inline float ndf_flux_mirror(float dw, float F, float sigma) {
+    if (dw == 0.f) return dw;
+    const float ad = fabsf(dw);
+    if constexpr (PEN == 3) {
+        if (!(ad <= sigma)) return 0.0f;
+    }
+    if constexpr (PEN == 4) {
+        if (ad > sigma) return 0.0f;
+    }
+    if constexpr (PEN == 5) {
+        if (!(ad <= 2.0f * sigma)) return 0.0f;
+    }
+    return -F;
+}
+
+// NDF with each edge's flux evaluated once: every point evaluates its +x,
+// +y, +z fluxes (the mirrored ones at the far boundary), shares the in-plane
+// ones through LDS (ring lanes add the tile's -x column and -y row) and
+// carries the z one to the next plane; the -x, -y, -z fluxes are the
+// neighbours' by ndf_flux_mirror, or at the near boundary (mirror index 1)
+// the point's own +x, +y, +z flux.  Same terms, same sum order as ndf_body.
 template <int DIM, int TY, typename I, int PEN>
+__device__ inline void ndf_shared_body(const DiffArgs<float>& a, const TileCoord& tc) {
+    const Geom& g = a.g;
+    constexpr int NT = TX * TY;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
+    const int x = tc.x0 + lx, y = tc.y0 + ly;
+    const bool own = x < g.nx && y < g.ny;
+    const int ox = own ? x : 0, oy = own ? y : 0;
+    const Ring<TY> r(tc, g, tid, NT);
+    __shared__ float Us[TY + 2][TX + 2];
+    __shared__ float FE[TY][TX + 1];   // [ly][lx + 1]: +x flux; column 0: x0 - 1
+    __shared__ float FN[TY + 1][TX];   // [ly + 1][lx]: +y flux; row 0: y0 - 1
+    const I sxy = (I)g.sxy, nx = (I)g.nx;
+    const I col = (I)oy * nx + (I)ox, rcol = (I)r.y * nx + (I)r.x;
+    const int sxp = mirror_p(ox, g.nx) - tc.x0 + 1, sxm = mirror_m(ox, g.nx) - tc.x0 + 1;
+    const int syp = mirror_p(oy, g.ny) - tc.y0 + 1, sym = mirror_m(oy, g.ny) - tc.y0 + 1;
+    const float sigma = a.sigma;
+
+    float uc = ldx<float>(a.U, (I)tc.z0 * sxy + col);
+    float um = 0.f, fu_prev = 0.f;
+    if (DIM == 3) {
+        um = ldx<float>(a.U, (I)mirror_m(tc.z0, g.nz) * sxy + col);
+        if (tc.z0 > 0) fu_prev = ndf_flux<PEN>(uc - um, sigma);   // +z flux of plane z0 - 1
+    }
+    double sdd = 0.0, suu = 0.0;
+    for (int z = tc.z0; z < tc.z1; ++z) {
+        float up = 0.f;
+        if (DIM == 3) up = (z + 1 < g.nz) ? ldx<float>(a.U, (I)(z + 1) * sxy + col) : um;
+        Us[ly + 1][lx + 1] = uc;
+        if (r.lane) Us[r.sy][r.sx] = ldx<float>(a.U, (I)z * sxy + rcol);
+        __syncthreads();
+        const float fe = ndf_flux<PEN>(Us[ly + 1][sxp] - uc, sigma);
+        const float fn = ndf_flux<PEN>(Us[syp][lx + 1] - uc, sigma);
+        const float dw = Us[ly + 1][sxm] - uc, ds = Us[sym][lx + 1] - uc;
+        float fu = 0.f;
+        if (DIM == 3) fu = ndf_flux<PEN>(up - uc, sigma);
+        FE[ly][lx + 1] = fe;
+        FN[ly + 1][lx] = fn;
+        if (r.lane) {
+            if (r.sx == 0) FE[r.sy - 1][0] = ndf_flux<PEN>(Us[r.sy][1] - Us[r.sy][0], sigma);
+            else if (r.sy == 0) FN[0][r.sx - 1] = ndf_flux<PEN>(Us[1][r.sx] - Us[0][r.sx], sigma);
+        }
+        __syncthreads();
+        if (own) {
+            const I i = (I)z * sxy + col;
+            const float fw = x == 0 ? fe : ndf_flux_mirror<PEN>(dw, FE[ly][lx], sigma);
+            const float fs = y == 0 ? fn : ndf_flux_mirror<PEN>(ds, FN[ly][lx], sigma);
+            float s = fe + fw + fn + fs;
+            if (DIM == 3) {
+                const float fd = z == 0 ? fu : ndf_flux_mirror<PEN>(um - uc, fu_prev, sigma);
+                s = s + fu + fd;
+            }
+            const float un = uc + a.tau * (a.lambda * s - (uc - ldx<float>(a.f, i)));
+            stnt<float>(a.Uout, i, un);
+            if (a.check) add_resid(un, uc, sdd, suu);
+        }
+        fu_prev = fu;
+        um = uc;
+        uc = up;
+    }
+    if (a.check) store_partials<TY>(a, tc, tid, sdd, suu);
+}
+
+// SH: the shared-flux body (penalties 1-5 with sigma > 0).
+template <int DIM, int TY, typename I, int PEN, bool SH>
 __global__ __launch_bounds__(TX * TY) void ndf_iter_kernel(DiffArgs<float> a) {
     if (solve_stopped(a.ctrl)) return;
-    ndf_body<DIM, TY, I, PEN>(a, tile_of_block(a.g, TY));
+    if constexpr (SH) ndf_shared_body<DIM, TY, I, PEN>(a, tile_of_block(a.g, TY));
+    else ndf_body<DIM, TY, I, PEN>(a, tile_of_block(a.g, TY));
 }
 
 // ---------------------------------------------------------------------------
@@ -416,17 +510,22 @@ __global__ __launch_bounds__(TX * TY) void d4_iter_kernel(DiffArgs<float> a) {
 enum Method { M_LLT, M_NDF, M_D4 };
 
 template <int DIM, int TY, typename I>
-const void* kernel_of(Method m, int penalty) {
+const void* kernel_of(Method m, int penalty, bool share) {
     switch (m) {
         case M_LLT: return (const void*)llt_iter_kernel<DIM, TY, I>;
         case M_NDF:
             switch (penalty) {
-                case 1: return (const void*)ndf_iter_kernel<DIM, TY, I, 1>;
-                case 2: return (const void*)ndf_iter_kernel<DIM, TY, I, 2>;
-                case 3: return (const void*)ndf_iter_kernel<DIM, TY, I, 3>;
-                case 4: return (const void*)ndf_iter_kernel<DIM, TY, I, 4>;
-                case 5: return (const void*)ndf_iter_kernel<DIM, TY, I, 5>;
-                default: return (const void*)ndf_iter_kernel<DIM, TY, I, 0>;
+                case 1: return share ? (const void*)ndf_iter_kernel<DIM, TY, I, 1, true>
+                                     : (const void*)ndf_iter_kernel<DIM, TY, I, 1, false>;
+                case 2: return share ? (const void*)ndf_iter_kernel<DIM, TY, I, 2, true>
+                                     : (const void*)ndf_iter_kernel<DIM, TY, I, 2, false>;
+                case 3: return share ? (const void*)ndf_iter_kernel<DIM, TY, I, 3, true>
+                                     : (const void*)ndf_iter_kernel<DIM, TY, I, 3, false>;
+                case 4: return share ? (const void*)ndf_iter_kernel<DIM, TY, I, 4, true>
+                                     : (const void*)ndf_iter_kernel<DIM, TY, I, 4, false>;
+                case 5: return share ? (const void*)ndf_iter_kernel<DIM, TY, I, 5, true>
+                                     : (const void*)ndf_iter_kernel<DIM, TY, I, 5, false>;
+                default: return (const void*)ndf_iter_kernel<DIM, TY, I, 0, false>;
             }
         default: return (const void*)d4_iter_kernel<DIM, TY, I>;
     }
@@ -435,7 +534,7 @@ const void* kernel_of(Method m, int penalty) {
 template <int DIM, int TY>
 void launch(Method m, const DiffArgs<float>& a, bool small, hipStream_t s) {
     dim3 block(TX, TY), grid(a.g.nblocks);
-    const void* k = small ? kernel_of<DIM, TY, uint32_t>(m, a.penalty) : kernel_of<DIM, TY, uint64_t>(m, a.penalty);
+    const void* k = small ? kernel_of<DIM, TY, uint32_t>(m, a.penalty, a.sigma > 0.f) : kernel_of<DIM, TY, uint64_t>(m, a.penalty, a.sigma > 0.f);
     DiffArgs<float> arg = a;
     void* args[] = {&arg};
     (void)hipLaunchKernel(k, grid, block, args, 0, s);
@@ -461,10 +560,12 @@ int diffusion_solve(Method m, const float* d_f, float* d_out, const DiffArgs<flo
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
     else if (ty == 8)
-        g = plan_geom(small ? kernel_of<3, 8, uint32_t>(m, proto.penalty) : kernel_of<3, 8, uint64_t>(m, proto.penalty),
+        g = plan_geom(small ? kernel_of<3, 8, uint32_t>(m, proto.penalty, proto.sigma > 0.f)
+                            : kernel_of<3, 8, uint64_t>(m, proto.penalty, proto.sigma > 0.f),
                       TX * 8, nx, ny, nz, 8, opts.zchunk);
     else
-        g = plan_geom(small ? kernel_of<3, 4, uint32_t>(m, proto.penalty) : kernel_of<3, 4, uint64_t>(m, proto.penalty),
+        g = plan_geom(small ? kernel_of<3, 4, uint32_t>(m, proto.penalty, proto.sigma > 0.f)
+                            : kernel_of<3, 4, uint64_t>(m, proto.penalty, proto.sigma > 0.f),
                       TX * 4, nx, ny, nz, 4, opts.zchunk);
     Arena ar;
     size_t oU[2], oCtrl, oPart;

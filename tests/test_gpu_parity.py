@@ -224,6 +224,10 @@ DIFF_CASES = [
     ("llt", {"lambda_rof": 0.03, "lambda_llt": 0.015, "iters": 20, "tau": 0.005, "tol": 0.0}),
     ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 20, "tau": 0.02, "penalty": 1, "tol": 0.0}),
     ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 20, "tau": 0.02, "penalty": 3, "tol": 0.0}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 20, "tau": 0.02, "penalty": 2, "tol": 0.0}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 20, "tau": 0.02, "penalty": 4, "tol": 0.0}),
+    ("ndf", {"lambda": 0.03, "sigma": 0.1, "iters": 20, "tau": 0.02, "penalty": 5, "tol": 0.0}),
+    ("ndf", {"lambda": 0.03, "sigma": -0.1, "iters": 20, "tau": 0.02, "penalty": 1, "tol": 0.0}),
     ("ndf", {"lambda": 0.03, "sigma": 0.0, "iters": 20, "tau": 0.02, "penalty": 1, "tol": 0.0}),
     ("d4", {"lambda": 1.0, "sigma": 0.05, "iters": 20, "tau": 1e-3, "tol": 0.0}),
 ]
