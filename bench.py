@@ -172,13 +172,11 @@ def secondary(D, torch):
     tgv_args = (0.02, 1.0, 2.0)   # lambda (reference tests' value), alpha1, alpha0; L2 = 12
 
     def objective(u):
-        # E = sum 2 lambda |grad+ u| + sum (u - f)^2 in FP64, replicate boundary (utils.c:89-117)
-        ud = u.double()
-        gx = torch.diff(ud, dim=2, append=ud[:, :, -1:])
-        gy = torch.diff(ud, dim=1, append=ud[:, -1:, :])
-        gz = torch.diff(ud, dim=0, append=ud[-1:, :, :])
-        e = (2 * tgv_args[0] * torch.sqrt(gx * gx + gy * gy + gz * gz)).sum() + ((ud - x.double()) ** 2).sum()
-        return float(e)
+        # E = sum 2 lambda |grad+ u| + sum (u - f)^2 in FP64, replicate boundary: the library's
+        # cilreg_tv_energy_dev(_f64), TV_energy3D of utils.c:89-117 summed in double
+        if u.dtype == torch.float64:
+            return D.tv_energy(u, x.double(), tgv_args[0], 1)
+        return D.tv_energy(u, x, tgv_args[0], 1)
 
     D.tgv(x, *tgv_args, 20, 12.0, 0.0)
     results = {}
@@ -346,28 +344,9 @@ def diffusion_section(D, torch, x):
     return out
 
 
-def batched_jobs(seed=4):
-    """64 uneven ellipsoid-phantom volumes, each extent drawn from U{64..320},
-    noise sigma U[0.02, 0.08]; even entries FGP-TV (lambda U[0.01, 0.05], 300
-    iterations, isotropic), odd entries ROF-TV (lambda U[0.01, 0.05], time
-    step 0.001, 1000 iterations); fixed iteration counts."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(64):
-        shape = tuple(int(v) for v in rng.integers(64, 321, size=3))
-        sigma = float(rng.uniform(0.02, 0.08))
-        lam = float(rng.uniform(0.01, 0.05))
-        if i % 2 == 0:
-            spec = {"method": "fgp", "iters": 300, "lambda": lam, "method_tv": 0, "nonneg": 0}
-        else:
-            spec = {"method": "rof", "iters": 1000, "lambda": lam, "tau": 0.001}
-        jobs.append((shape, (20, sigma, 9000 + i), spec))
-    return jobs
-
-
 def batched_section(D, B, torch):
     """Plain (one solve after another) vs one autotuned batched hipGraph launch, FP32 and BF16 / FP16 duals."""
+    from ccpi.supp.workloads import batched_jobs
     jobs = batched_jobs()
     xs = [B.phantom(shape, *ph) for shape, ph, _ in jobs]
     specs = [dict(sp, input=x) for (_, _, sp), x in zip(jobs, xs)]
@@ -433,38 +412,11 @@ def batched_section(D, B, torch):
     }
 
 
-def request_stream(rate=200.0, seconds=60.0, seed=5):
-    """Poisson arrivals; each request ROF/FGP/PD/TGV uniformly, a 2D slice
-    (square, side U{256..2048}) with p = 0.5 else a 3D cube (side U{64..384}),
-    generated phantom with noise sigma U[0.02, 0.08], lambda U[0.01, 0.05],
-    iterations U{100..500}."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    t = 0.0
-    reqs = []
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > seconds:
-            break
-        method = ["rof", "fgp", "pd", "tgv"][int(rng.integers(0, 4))]
-        if rng.random() < 0.5:
-            side = int(rng.integers(256, 2049))
-            shape = (side, side)
-        else:
-            side = int(rng.integers(64, 385))
-            shape = (side, side, side)
-        sigma = float(rng.uniform(0.02, 0.08))
-        spec = {"method": method, "shape": shape, "phantom": (20, sigma, 20000 + len(reqs)),
-                "lambda": float(rng.uniform(0.01, 0.05)), "iters": int(rng.integers(100, 501)),
-                "tau": 0.001, "L": 12.0, "alpha1": 1.0, "alpha0": 2.0, "L2": 12.0}
-        reqs.append((t, spec))
-    return reqs
-
-
 def served_section(D, B, torch, window):
     """The request stream replayed in real time through the server (FP32 and
     BF16 duals), and the same requests solved plainly one after another."""
     import numpy as np
+    from ccpi.supp.workloads import request_stream, solve_alone
     reqs = request_stream(seconds=window)
     out = {"workload": f"Poisson 200 req/s for {window:g} s ({len(reqs)} requests), ROF/FGP/PD/TGV, "
                        "2D 256^2..2048^2 or 3D 64^3..384^3, 100..500 iterations, HBM budget 200 GB"}
@@ -501,15 +453,7 @@ def served_section(D, B, torch, window):
     vi = 0
     for _, sp in reqs:
         x = B.phantom(sp["shape"], *sp["phantom"])
-        m = sp["method"]
-        if m == "rof":
-            _, info = D.rof(x, sp["lambda"], sp["iters"], sp["tau"], 0.0)
-        elif m == "fgp":
-            _, info = D.fgp(x, sp["lambda"], sp["iters"], 0.0, 0, 0)
-        elif m == "pd":
-            _, info = D.pd(x, sp["lambda"], sp["iters"], 0.0, sp["L"], 0, 0)
-        else:
-            _, info = D.tgv(x, sp["lambda"], sp["alpha1"], sp["alpha0"], sp["iters"], sp["L2"], 0.0)
+        _, info = solve_alone(D, sp, x)
         vi += x.numel() * int(info[0])
         del x
     torch.cuda.synchronize()

@@ -56,6 +56,8 @@ def _lib():
     lib.cilreg_server_poll.restype = ctypes.c_int
     lib.cilreg_server_status.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
     lib.cilreg_server_status.restype = ctypes.c_int
+    lib.cilreg_server_cancel.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    lib.cilreg_server_cancel.restype = ctypes.c_int
     lib.cilreg_server_destroy.argtypes = [ctypes.c_void_p]
     lib.cilreg_server_destroy.restype = None
     return lib
@@ -212,6 +214,14 @@ class Server:
         if rc < 0:
             raise RuntimeError(f"job {ticket} failed: {last_error()}")
         return out, np.array([info[0], info[1]], dtype=np.float32)
+
+    def cancel(self, ticket):
+        """Withdraw a job in any state; its device memory is released."""
+        if ticket not in self._keep:
+            raise KeyError(ticket)
+        if self._lib.cilreg_server_cancel(self._h, int(ticket)) != 0:
+            raise RuntimeError(f"cilreg_server_cancel failed: {last_error()}")
+        self._keep.pop(ticket, None)
 
     def close(self):
         if self._h:

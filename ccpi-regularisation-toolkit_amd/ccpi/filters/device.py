@@ -234,6 +234,29 @@ def tgv_f64(x, lam, alpha1, alpha0, iters, L2=12.0, tol=0.0, out=None, **kw):
                 make_opts(**kw))
 
 
+def tv_energy(u, u0, lam, type=1):
+    """The ROF objective in FP64 (TV_energy2D/3D of the reference's utils.c:64-117
+    summed in double): type 1 = sum 2 lam |grad+ u| + sum (u - u0)^2, type 2 =
+    the gradient term alone.  u, u0: contiguous CUDA tensors of one shape,
+    both float32 or both float64."""
+    import torch
+    if not (u.is_cuda and u0.is_cuda and u.shape == u0.shape and u.dtype == u0.dtype and u.is_contiguous()
+            and u0.is_contiguous() and u.dtype in (torch.float32, torch.float64) and u.device == u0.device):
+        raise ValueError("u and u0 must be contiguous CUDA tensors of one shape, device and dtype (float32/float64)")
+    lib = require_gpu_lib()
+    name = "cilreg_tv_energy_dev" if u.dtype == torch.float32 else "cilreg_tv_energy_dev_f64"
+    f = getattr(lib, name)
+    f.argtypes = [_vp, _vp, _d, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_double), _vp]
+    f.restype = _i
+    e = ctypes.c_double()
+    with torch.cuda.device(u.device):
+        rc = f(u.data_ptr(), u0.data_ptr(), float(lam), int(type), *_dims(u), ctypes.byref(e),
+               ctypes.c_void_p(torch.cuda.current_stream(u.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"{name} failed: {last_error()}")
+    return e.value
+
+
 # ---------------------------------------------------------------------------
 # warm start (include/cilreg_ext.h: cilreg_*_dev_ws)
 # ---------------------------------------------------------------------------

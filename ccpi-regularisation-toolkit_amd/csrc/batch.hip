@@ -625,6 +625,7 @@ public:
     virtual int step(int levels, int* n_active, int* n_pending) = 0;
     virtual int poll(long long ticket, float* info, float* d_dst) = 0;
     virtual int status(long long ticket) const = 0;
+    virtual int cancel(long long ticket) = 0;
     virtual int run_batch(cilreg_job* jobs, int n) = 0;
 };
 
@@ -719,6 +720,26 @@ public:
         auto it = all_.find(ticket);
         if (it == all_.end() || it->second->state == J_FAILED) return -1;
         return it->second->state;   // J_PENDING 0, J_ACTIVE 1, J_DONE 2
+    }
+
+    // Removes a job in any state; its device memory returns to the pool once
+    // the kernels already queued (which may still use it) have finished.
+    int cancel(long long ticket) override {
+        auto it = all_.find(ticket);
+        if (it == all_.end()) { set_error("unknown ticket %lld", ticket); return -1; }
+        EJob* j = it->second.get();
+        if (j->state == J_PENDING) {
+            pending_.erase(std::find(pending_.begin(), pending_.end(), j));
+        } else {
+            CILREG_CK(hipStreamSynchronize(s_));
+            if (j->state == J_ACTIVE) {
+                active_.erase(std::find(active_.begin(), active_.end(), j));
+                tables_dirty_ = true;
+            }
+            if (j->state == J_ACTIVE || j->state == J_DONE) used_ -= (double)j->bytes;
+        }
+        all_.erase(it);
+        return 0;
     }
 
     int run_batch(cilreg_job* jobs, int n) override {
@@ -1346,6 +1367,13 @@ int cilreg_server_poll(cilreg_server* srv, long long ticket, float* infovector, 
     std::lock_guard<std::mutex> lk(srv->mu);
     CILREG_CK(hipSetDevice(srv->dev));
     return srv->eng->poll(ticket, infovector, d_dst);
+}
+
+int cilreg_server_cancel(cilreg_server* srv, long long ticket) {
+    if (!srv) return -1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    CILREG_CK(hipSetDevice(srv->dev));
+    return srv->eng->cancel(ticket);
 }
 
 int cilreg_server_status(cilreg_server* srv, long long ticket) {

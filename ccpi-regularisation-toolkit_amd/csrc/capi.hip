@@ -37,8 +37,8 @@ bool host_ptrs_ok(const void* in, const void* out, const void* info) {
 // device buffers, download the result into `out` unless `keep_out_on_zero_iter`
 // and iter == 0.
 template <class Solve>
-int host_call(int gpu_device, const float* in, float* out, float* info, size_t n, bool upload_out, bool skip_download,
-              Solve&& solve) {
+int host_call_impl(int gpu_device, const float* in, float* out, float* info, size_t n, bool upload_out,
+                   bool skip_download, Solve&& solve) {
     if (!host_ptrs_ok(in, out, info)) return -1;
     if (select_device(gpu_device) != 0) return -1;
     hipStream_t s = device_stream();
@@ -68,8 +68,8 @@ int host_call(int gpu_device, const float* in, float* out, float* info, size_t n
 // host_call with a second input uploaded beside the first (FGP-dTV's
 // reference image); solve receives both device pointers.
 template <class Solve>
-int host_call_ref(int gpu_device, const float* in, const float* ref, float* out, float* info, size_t n,
-                  bool upload_out, bool skip_download, Solve&& solve) {
+int host_call_ref_impl(int gpu_device, const float* in, const float* ref, float* out, float* info, size_t n,
+                       bool upload_out, bool skip_download, Solve&& solve) {
     if (!host_ptrs_ok(in, out, info)) return -1;
     if (!ref) { set_error("null reference image pointer"); return -1; }
     if (select_device(gpu_device) != 0) return -1;
@@ -96,6 +96,30 @@ int host_call_ref(int gpu_device, const float* in, const float* ref, float* out,
     info[0] = (float)done;
     info[1] = re;
     return 0;
+}
+
+// The drop-in entry points own no device memory between calls, like the
+// reference's (which allocate and free everything inside each call): the
+// workspace goes back to the device once the call's buffers are released,
+// unless CILREG_KEEP_CACHE=1 keeps it cached for the next call.
+void release_after_host_call() {
+    if (!keep_cache_after_host_calls()) (void)pool_release_idle();
+}
+
+template <class Solve>
+int host_call(int gpu_device, const float* in, float* out, float* info, size_t n, bool upload_out, bool skip_download,
+              Solve&& solve) {
+    const int rc = host_call_impl(gpu_device, in, out, info, n, upload_out, skip_download, solve);
+    release_after_host_call();
+    return rc;
+}
+
+template <class Solve>
+int host_call_ref(int gpu_device, const float* in, const float* ref, float* out, float* info, size_t n,
+                  bool upload_out, bool skip_download, Solve&& solve) {
+    const int rc = host_call_ref_impl(gpu_device, in, ref, out, info, n, upload_out, skip_download, solve);
+    release_after_host_call();
+    return rc;
 }
 
 bool dims_ok(int nx, int ny, int nz) {
@@ -129,6 +153,14 @@ int cilreg_last_profile(double* mean_ms, double* min_ms, int* count) {
 }
 
 int cilreg_release_cache(void) { return pool_release_idle(); }
+
+int cilreg_pool_stats(size_t* busy_bytes, size_t* idle_bytes) {
+    if (!busy_bytes || !idle_bytes) { set_error("null pointer argument"); return -1; }
+    pool_stats(busy_bytes, idle_bytes);
+    return 0;
+}
+
+void cilreg_inject_alloc_failure(int n) { inject_alloc_failure(n); }
 
 // ---------------------------------------------------------------------------
 // reference ABI

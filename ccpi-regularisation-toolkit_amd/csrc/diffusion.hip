@@ -137,8 +137,9 @@ __device__ inline void ndf_body(const DiffArgs<float>& a, const TileCoord& tc) {
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const I col = (I)oy * nx + (I)ox, rcol = (I)r.y * nx + (I)r.x;
     // mirrored neighbour slots of the own point
-    const int sxp = mirror_p(ox, g.nx) - tc.x0 + 1, sxm = mirror_m(ox, g.nx) - tc.x0 + 1;
-    const int syp = mirror_p(oy, g.ny) - tc.y0 + 1, sym = mirror_m(oy, g.ny) - tc.y0 + 1;
+    // threads outside the volume read their own slot (results not stored)
+    const int sxp = own ? mirror_p(x, g.nx) - tc.x0 + 1 : lx + 1, sxm = own ? mirror_m(x, g.nx) - tc.x0 + 1 : lx + 1;
+    const int syp = own ? mirror_p(y, g.ny) - tc.y0 + 1 : ly + 1, sym = own ? mirror_m(y, g.ny) - tc.y0 + 1 : ly + 1;
 
     float uc = ldx<float>(a.U, (I)tc.z0 * sxy + col);
     float um = 0.f;
@@ -221,8 +222,9 @@ __device__ inline void ndf_shared_body(const DiffArgs<float>& a, const TileCoord
     __shared__ float FN[TY + 1][TX];   // [ly + 1][lx]: +y flux; row 0: y0 - 1
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const I col = (I)oy * nx + (I)ox, rcol = (I)r.y * nx + (I)r.x;
-    const int sxp = mirror_p(ox, g.nx) - tc.x0 + 1, sxm = mirror_m(ox, g.nx) - tc.x0 + 1;
-    const int syp = mirror_p(oy, g.ny) - tc.y0 + 1, sym = mirror_m(oy, g.ny) - tc.y0 + 1;
+    // threads outside the volume read their own slot (results not stored)
+    const int sxp = own ? mirror_p(x, g.nx) - tc.x0 + 1 : lx + 1, sxm = own ? mirror_m(x, g.nx) - tc.x0 + 1 : lx + 1;
+    const int syp = own ? mirror_p(y, g.ny) - tc.y0 + 1 : ly + 1, sym = own ? mirror_m(y, g.ny) - tc.y0 + 1 : ly + 1;
     const float sigma = a.sigma;
 
     float uc = ldx<float>(a.U, (I)tc.z0 * sxy + col);
@@ -329,8 +331,9 @@ __device__ inline void llt_body(const DiffArgs<float>& a, const TileCoord& tc) {
     __shared__ float L1s[TY + 2][TX + 2], L2s[TY + 2][TX + 2], R1s[TY + 2][TX + 2], R2s[TY + 2][TX + 2];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const I col = (I)oy * nx + (I)ox;
-    const int sxp = mirror_p(ox, g.nx) - tc.x0 + 1, sxm = mirror_m(ox, g.nx) - tc.x0 + 1;
-    const int syp = mirror_p(oy, g.ny) - tc.y0 + 1, sym = mirror_m(oy, g.ny) - tc.y0 + 1;
+    // threads outside the volume read their own slot (results not stored)
+    const int sxp = own ? mirror_p(x, g.nx) - tc.x0 + 1 : lx + 1, sxm = own ? mirror_m(x, g.nx) - tc.x0 + 1 : lx + 1;
+    const int syp = own ? mirror_p(y, g.ny) - tc.y0 + 1 : ly + 1, sym = own ? mirror_m(y, g.ny) - tc.y0 + 1 : ly + 1;
 
     float r3m = 0.f, l3m = 0.f, l3c = 0.f;   // D_rof,3 and D_llt,3 of the own column below / here
     if (DIM == 3) {
@@ -469,8 +472,9 @@ __device__ inline void d4_body(const DiffArgs<float>& a, const TileCoord& tc) {
     __shared__ float Ws[TY + 2][TX + 2];
     const I sxy = (I)g.sxy, nx = (I)g.nx;
     const I col = (I)oy * nx + (I)ox;
-    const int sxp = mirror_p(ox, g.nx) - tc.x0 + 1, sxm = mirror_m(ox, g.nx) - tc.x0 + 1;
-    const int syp = mirror_p(oy, g.ny) - tc.y0 + 1, sym = mirror_m(oy, g.ny) - tc.y0 + 1;
+    // threads outside the volume read their own slot (results not stored)
+    const int sxp = own ? mirror_p(x, g.nx) - tc.x0 + 1 : lx + 1, sxm = own ? mirror_m(x, g.nx) - tc.x0 + 1 : lx + 1;
+    const int syp = own ? mirror_p(y, g.ny) - tc.y0 + 1 : ly + 1, sym = own ? mirror_m(y, g.ny) - tc.y0 + 1 : ly + 1;
 
     float wm = 0.f;   // W of the own column at the mirrored z - 1
     if (DIM == 3) wm = d4_w<DIM, I>(a, ox, oy, mirror_m(tc.z0, g.nz));

@@ -34,16 +34,44 @@ int select_device(int dev) {
     return 0;
 }
 
+// Streams are per host thread and device, so calls from different threads
+// run concurrently as the reference's default-stream calls from different
+// threads do.  A thread's streams go back to a free list when it exits (for
+// the next thread to take); they are never destroyed, which keeps thread and
+// process teardown free of HIP calls.
+namespace {
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_free_streams;
+
+struct ThreadStreams {
+    hipStream_t s[64] = {};
+    ~ThreadStreams() {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (int d = 0; d < 64; ++d)
+            if (s[d]) g_free_streams.push_back({d, s[d]});
+    }
+};
+}  // namespace
+
 hipStream_t device_stream() {
-    static std::mutex mu;
-    static hipStream_t streams[64] = {};
+    static thread_local ThreadStreams ts;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    if (!streams[dev]) {
-        if (hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess) streams[dev] = nullptr;
+    if (!ts.s[dev]) {
+        {
+            std::lock_guard<std::mutex> lk(g_stream_mu);
+            for (size_t i = 0; i < g_free_streams.size(); ++i) {
+                if (g_free_streams[i].first == dev) {
+                    ts.s[dev] = g_free_streams[i].second;
+                    g_free_streams[i] = g_free_streams.back();
+                    g_free_streams.pop_back();
+                    break;
+                }
+            }
+        }
+        if (!ts.s[dev] && hipStreamCreateWithFlags(&ts.s[dev], hipStreamNonBlocking) != hipSuccess) ts.s[dev] = nullptr;
     }
-    return streams[dev];
+    return ts.s[dev];
 }
 
 hipStream_t caller_stream(void* stream) {
@@ -98,7 +126,14 @@ int free_idle_locked(int dev) {
 }
 }  // namespace
 
+// Failure injection for tests: the n-th next pool_alloc on this thread fails
+// as an exhausted device would (0 = off).
+static thread_local int g_fail_alloc_in = 0;
+
+void inject_alloc_failure(int n) { g_fail_alloc_in = n > 0 ? n : 0; }
+
 void* pool_alloc(size_t bytes) {
+    if (g_fail_alloc_in > 0 && --g_fail_alloc_in == 0) return nullptr;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lk(g_pool_mu);
@@ -149,6 +184,26 @@ int pool_release_idle() {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     free_idle_locked(dev);
     return 0;
+}
+
+void pool_stats(size_t* busy, size_t* idle) {
+    int dev = 0;
+    *busy = *idle = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (const PoolBlock& b : g_pool) {
+        if (b.dev != dev) continue;
+        (b.busy ? *busy : *idle) += b.bytes;
+    }
+}
+
+bool keep_cache_after_host_calls() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("CILREG_KEEP_CACHE");
+        v = (e && e[0] && e[0] != '0') ? 1 : 0;
+    }
+    return v == 1;
 }
 
 ProfileStats& thread_profile() {

@@ -30,7 +30,8 @@ const char* last_error();
 // hipSetDevice(dev) after checking the device count; -1 with a message otherwise.
 int select_device(int dev);
 
-// The library's own stream on the current device (default priority, created once).
+// The library's own stream for this host thread on the current device
+// (default priority, created once per thread and device).
 hipStream_t device_stream();
 // The stream a device-API call runs on: the caller's, or for NULL the
 // library's stream ordered after the work queued on the legacy default stream.
@@ -42,6 +43,13 @@ hipStream_t caller_stream(void* stream);
 void* pool_alloc(size_t bytes);
 void pool_free(void* p);
 int pool_release_idle();
+// Bytes of the current device's pool held by running solves / kept idle.
+void pool_stats(size_t* busy, size_t* idle);
+// Test hook: the n-th next pool_alloc on this thread fails (0 = off).
+void inject_alloc_failure(int n);
+// The drop-in host-pointer entry points free their device memory before
+// returning, as the reference's do, unless CILREG_KEEP_CACHE=1.
+bool keep_cache_after_host_calls();
 
 // One device allocation carved into aligned pieces; returned to the pool on
 // destruction, so every exit path of a solve releases its memory.

@@ -88,6 +88,20 @@ CILREG_API int cilreg_last_profile(double *mean_ms, double *min_ms, int *count);
  * set CILREG_NO_CACHE=1 to allocate and free per call like the reference). */
 CILREG_API int cilreg_release_cache(void);
 
+/* Device memory of the library's pool on the current device: bytes held by
+ * running solves (busy) and kept cached for reuse (idle).  The drop-in
+ * *_GPU_main entry points return their workspace to the device before they
+ * return, as the reference's do (CILREG_KEEP_CACHE=1 keeps it cached instead),
+ * so after a completed or failed drop-in call both are 0 unless other calls
+ * of the process hold memory. */
+CILREG_API int cilreg_pool_stats(size_t *busy_bytes, size_t *idle_bytes);
+
+/* Test hook for failure handling: the n-th next device allocation the
+ * library requests on this host thread fails as an exhausted device would
+ * (n = 0 turns it off).  Every entry point then returns -1 with the
+ * allocation error and releases what it had allocated. */
+CILREG_API void cilreg_inject_alloc_failure(int n);
+
 /* Device-resident solves.  d_input and d_output are device arrays of
  * dimX*dimY*dimZ floats (2D: dimZ = 1); infovector is a host float[2]
  * receiving [iterations, last residual] as in the reference (NULL skips it).
@@ -132,6 +146,18 @@ CILREG_API int cilreg_pd_dev_f64(const double *d_input, double *d_output, double
 CILREG_API int cilreg_tgv_dev_f64(const double *d_input, double *d_output, double *infovector, double lambda, double alpha1,
                        double alpha0, int iter, double L2, double epsil, int dimX, int dimY, int dimZ,
                        const cilreg_opts *opts, void *stream);
+
+/* The ROF objective in FP64 (the "final objective" of a precision study):
+ * TV_energy2D/3D of src/Core/regularisers_CPU/utils.c:64-117, i.e.
+ * type 1: sum 2*lambda*|grad+ u| + sum (u - u0)^2, type 2: the gradient term
+ * alone, forward differences with a replicate boundary; every term formed and
+ * summed in double in a fixed order (deterministic), where the reference
+ * sums serially in float.  d_U, d_U0: device arrays of dimX*dimY*dimZ values
+ * (2D: dimZ = 1); *energy is a host double. */
+CILREG_API int cilreg_tv_energy_dev(const float *d_U, const float *d_U0, double lambda, int type, int dimX, int dimY,
+                                    int dimZ, double *energy, void *stream);
+CILREG_API int cilreg_tv_energy_dev_f64(const double *d_U, const double *d_U0, double lambda, int type, int dimX,
+                                        int dimY, int dimZ, double *energy, void *stream);
 
 /* Achievable HBM bandwidth of the current device, from streaming kernels
  * over arrays of `bytes` each (>= 1 MiB), the mean of `reps` timed launches:
@@ -203,6 +229,10 @@ CILREG_API int cilreg_server_poll(cilreg_server *srv, long long ticket, float *i
 /* 0 waiting for admission, 1 running, 2 done (collect with cilreg_server_poll),
  * -1 failed (e.g. larger than the budget; poll reports the error) or unknown. */
 CILREG_API int cilreg_server_status(cilreg_server *srv, long long ticket);
+/* Withdraws a job in any state (waiting, running, or done and not yet
+ * polled): it is removed from the server and its device memory released;
+ * the other jobs are unaffected.  0 on success, -1 for an unknown ticket. */
+CILREG_API int cilreg_server_cancel(cilreg_server *srv, long long ticket);
 CILREG_API void cilreg_server_destroy(cilreg_server *srv);
 
 #ifdef __cplusplus

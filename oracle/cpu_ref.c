@@ -75,10 +75,20 @@ static REAL rel_change(const REAL *u, const REAL *u_prev, idx_t n)
  * (not consecutive) and the loop breaks once it exceeds 3. */
 typedef struct { int count; REAL re; } stop_state;
 
-static int stop_check(stop_state *st, const REAL *u, const REAL *u_prev, idx_t n, REAL epsil)
+/* Test hook: with forced_stop >= 0 the rule fires at the check of iteration
+ * forced_stop and at no other (the residual and count are still computed).
+ * Parity tests use it to obtain the output of a solve that stopped at a given
+ * iteration, whatever the residual sums decide.  -1 (the default) is the
+ * reference's rule. */
+static int forced_stop = -1;
+
+void FN(cr_set_forced_stop)(int it) { forced_stop = it; }
+
+static int stop_check(stop_state *st, const REAL *u, const REAL *u_prev, idx_t n, REAL epsil, int it)
 {
     st->re = rel_change(u, u_prev, n);
     if (st->re < epsil) st->count++;
+    if (forced_stop >= 0) return it == forced_stop;
     return st->count > 3;
 }
 
@@ -232,7 +242,7 @@ REAL FN(cr_rof)(const REAL *Input, REAL *Output, REAL *infovector, const REAL *l
         rof_dir(Output, D2, dimX, dimY, dimZ, 1, ndim, ROF_EPS);
         if (ndim == 3) rof_dir(Output, D3, dimX, dimY, dimZ, 2, ndim, ROF_EPS);
         rof_update(D1, D2, D3, Output, Input, lambdaPar, lambda_is_arr, tau, dimX, dimY, dimZ, ndim);
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     free(D1); free(D2); free(D3); free(prev);
     infovector[0] = (REAL)it;
@@ -332,7 +342,7 @@ REAL FN(cr_fgp)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPa
         /* The 2D branch stores P_old before the residual check, the 3D branch
          * after it (FGP_TV_core.c:84-98 vs :142-159); neither order changes
          * the output, since a break returns D. */
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
         for (d = 0; d < ndim; d++) copy_field(P[d], Pold[d], n);
         tk = tkp1;
     }
@@ -473,7 +483,7 @@ REAL FN(cr_dtv)(const REAL *Input, const REAL *InputRef, REAL *Output, REAL *inf
             for (d = 0; d < ndim; d++) R[d][t] = P[d][t] + m * (P[d][t] - Pold[d][t]);
         for (d = 0; d < ndim; d++) copy_field(P[d], Pold[d], n);
         tk = tkp1;
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     for (d = 0; d < 3; d++) { free(P[d]); free(Pold[d]); free(R[d]); free(B[d]); }
     free(prev);
@@ -545,7 +555,7 @@ REAL FN(cr_pd)(const REAL *Input, REAL *U, REAL *infovector, REAL lambdaPar, int
                     U[c] = (REAL)((U[c] - tau * dv + lt * Input[c]) / (1.0 + lt));
                 }
         }
-        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, U, Uold, n, epsil)) break;
+        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, U, Uold, n, epsil, it)) break;
         /* over-relaxation, PD_TV_core.c:197-205 */
 #pragma omp parallel for
         for (t = 0; t < n; t++) U[t] += theta * (U[t] - Uold[t]);
@@ -669,7 +679,7 @@ REAL FN(cr_sb)(const REAL *Input, REAL *Output, REAL *infovector, REAL mu, int i
         copy_field(Output, prev, n);
         sb_sweep(Output, Input, prev, Dd, Bb, dimX, dimY, dimZ, ndim, lam, mu_inv);
         sb_shrink_bregman(Output, Dd, Bb, dimX, dimY, dimZ, ndim, methodTV == 1, lam);
-        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, Output, prev, n, epsil)) break;
+        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     for (d = 0; d < 3; d++) { free(Dd[d]); free(Bb[d]); }
     free(prev);
@@ -849,7 +859,7 @@ REAL FN(cr_tgv)(const REAL *U0, REAL *U, REAL *infovector, REAL lambda, REAL alp
         tgv_dual_p(&g, U, V, P, sigma, alpha1);
         tgv_dual_q(&g, V, Q, sigma, alpha0);
         tgv_primal(&g, U, Uold, U0, V, Vold, P, Q, lambda, tau);
-        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, U, Uold, n, epsil)) break;
+        if ((epsil != 0) && (it % 5 == 0) && stop_check(&st, U, Uold, n, epsil, it)) break;
     }
     for (a = 0; a < 3; a++) { free(P[a]); free(V[a]); free(Vold[a]); }
     for (a = 0; a < 6; a++) free(Q[a]);
@@ -948,7 +958,7 @@ REAL FN(cr_llt)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaRO
         llt_second(Output, L[0], L[1], L[2], dimX, dimY, dimZ, ndim);
         llt_update((const REAL *const *)L, (const REAL *const *)R, Output, Input, lambdaROF, lambdaLLT, tau, dimX,
                    dimY, dimZ, ndim);
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     for (d = 0; d < 3; d++) { free(L[d]); free(R[d]); }
     free(prev);
@@ -1052,7 +1062,7 @@ static REAL ndf_run(const REAL *Input, REAL *Output, REAL *infovector, REAL lamb
             copy_field(Output, tmp, n);
             ndf_step(Input, tmp, Output, lambdaPar, sigma2, tau, linear, penaltytype, dimX, dimY, dimZ, ndim, 0);
         }
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     free(prev);
     free(tmp);
@@ -1182,7 +1192,7 @@ REAL FN(cr_d4)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar
         if (check) copy_field(Output, prev, n);
         d4_wlap(Output, W, sigma2, dimX, dimY, dimZ, ndim);
         d4_update(W, Output, Input, lambdaPar, tau, dimX, dimY, dimZ, ndim);
-        if (check && stop_check(&st, Output, prev, n, epsil)) break;
+        if (check && stop_check(&st, Output, prev, n, epsil, it)) break;
     }
     free(W);
     free(prev);
@@ -1193,13 +1203,15 @@ REAL FN(cr_d4)(const REAL *Input, REAL *Output, REAL *infovector, REAL lambdaPar
 
 /* ------------------------------------------------------------------------ */
 /* Objective of the ROF model in FP64, used for the "final objective" of the */
-/* precision study: E = sum 2*lambda*|grad+ u| + sum (u - u0)^2 with the     */
-/* replicate boundary of utils.c:64-117 (TV_energy2D/3D).  Accumulated in    */
-/* double in index order, so it is not bit-equal to the reference's serial  */
-/* float sum, by design.                                                     */
+/* precision study: TV_energy2D/3D of utils.c:64-117 with the same replicate */
+/* boundary (the forward difference at the last index is 0) and the same    */
+/* `type` (1: sum 2*lambda*|grad+ u| + sum (u - u0)^2; 2: the gradient term  */
+/* alone), accumulated in double in index order instead of the reference's  */
+/* serial float sum.  A 2D image is dimZ = 1 (its z difference is then 0,   */
+/* which is TV_energy2D).                                                    */
 /* ------------------------------------------------------------------------ */
 #ifndef CR_DOUBLE
-double cr_tv_energy_f64(const float *U, const float *U0, double lambda, int dimX, int dimY, int dimZ)
+double cr_tv_energy_f64(const float *U, const float *U0, double lambda, int type, int dimX, int dimY, int dimZ)
 {
     idx_t sxy = (idx_t)dimX * dimY, k, j, i;
     double e_grad = 0, e_data = 0;
@@ -1214,6 +1226,6 @@ double cr_tv_energy_f64(const float *U, const float *U0, double lambda, int dimX
                 e_grad += 2.0 * lambda * sqrt(gx * gx + gy * gy + gz * gz);
                 e_data += (u - U0[c]) * (u - U0[c]);
             }
-    return e_grad + e_data;
+    return type == 2 ? e_grad : e_grad + e_data;
 }
 #endif

@@ -240,12 +240,57 @@ def reference():
     return _cache[key]
 
 
-def tv_energy_f64(u, u0, lam):
+def tv_energy_f64(u, u0, lam, type=1):
+    """The FP64 objective of cpu_ref.c (TV_energy2D/3D of utils.c:64-117 in double)."""
     lib = cpu_ref().lib
     f = lib.cr_tv_energy_f64
-    f.argtypes = [_f32p, _f32p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    f.argtypes = [_f32p, _f32p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     f.restype = ctypes.c_double
     u = np.ascontiguousarray(u, dtype=np.float32)
     u0 = np.ascontiguousarray(u0, dtype=np.float32)
     d = _dims(u)
-    return f(u.ctypes.data_as(_f32p), u0.ctypes.data_as(_f32p), lam, *d)
+    return f(u.ctypes.data_as(_f32p), u0.ctypes.data_as(_f32p), lam, type, *d)
+
+
+def reference_tv_energy(u, u0, lam, type=1):
+    """TV_energy2D / TV_energy3D of the reference library (utils.c:64-117):
+    a serial float sum.  None when the reference library is not built."""
+    ref = reference()
+    if ref is None:
+        return None
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    u0 = np.ascontiguousarray(u0, dtype=np.float32)
+    e = np.zeros(1, dtype=np.float32)
+    i = ctypes.c_int
+    if u.ndim == 2:
+        f = ref.lib.TV_energy2D
+        f.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, i, i, i]
+        f.restype = ctypes.c_float
+        f(u.ctypes.data_as(_f32p), u0.ctypes.data_as(_f32p), e.ctypes.data_as(_f32p), lam, type, u.shape[1],
+          u.shape[0])
+    else:
+        f = ref.lib.TV_energy3D
+        f.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, i, i, i, i]
+        f.restype = ctypes.c_float
+        f(u.ctypes.data_as(_f32p), u0.ctypes.data_as(_f32p), e.ctypes.data_as(_f32p), lam, type, u.shape[2],
+          u.shape[1], u.shape[0])
+    return float(e[0])
+
+
+class forced_stop:
+    """Within the block the restatement's stopping rule fires at the check
+    of iteration ``it`` and at no other (``cr_set_forced_stop``): the output of
+    a solve that stopped there, whatever its residual sums decide."""
+
+    def __init__(self, it, prec="f32"):
+        self.it = int(it)
+        self.f = getattr(cpu_ref(prec).lib, f"cr_set_forced_stop_{prec}")
+        self.f.argtypes = [ctypes.c_int]
+        self.f.restype = None
+
+    def __enter__(self):
+        self.f(self.it)
+        return self
+
+    def __exit__(self, *exc):
+        self.f(-1)

@@ -133,16 +133,55 @@ def test_fp64_build_agrees_with_fp32(method):
     assert err < 1e-5, err
 
 
-def test_tv_energy_matches_direct_sum():
-    u = RNG.random((4, 5, 6)).astype(np.float32)
-    u0 = RNG.random((4, 5, 6)).astype(np.float32)
-    e = oracle.tv_energy_f64(u, u0, 0.1)
-    ud = u.astype(np.float64)
-    gx = np.diff(ud, axis=2, append=ud[:, :, -1:])
-    gy = np.diff(ud, axis=1, append=ud[:, -1:, :])
-    gz = np.diff(ud, axis=0, append=ud[-1:, :, :])
-    direct = np.sum(2 * 0.1 * np.sqrt(gx ** 2 + gy ** 2 + gz ** 2)) + np.sum((ud - u0) ** 2)
-    assert abs(e - direct) <= 1e-9 * abs(direct)
+ENERGY_SHAPES = [(37, 53), (11, 13, 17), (9, 16, 23)]
+
+
+@pytest.mark.skipif(oracle.reference() is None, reason="reference library not built (make -C oracle ref)")
+@pytest.mark.parametrize("shape", ENERGY_SHAPES, ids=["2d-37x53", "3d-11x13x17", "3d-9x16x23"])
+@pytest.mark.parametrize("etype", [1, 2], ids=["grad+data", "grad"])
+def test_tv_energy_matches_reference(shape, etype):
+    """cr_tv_energy_f64 against the reference's TV_energy2D/3D (utils.c:64-117).
+
+    The reference sums n positive terms serially in float; every term is
+    itself within a few ulps (powf, sqrtf, the products), so its total is
+    within (n + 8) * 2^-24 of the exact sum, relatively (first-order bound for
+    a float sum of non-negative terms).  The FP64 sum is exact to ~1e-15.
+    Non-cubic shapes make a transposed axis, a wrong boundary rule or a
+    dropped factor miss by far more than that bound; type 2 pins the gradient
+    term alone, type 1 adds the fidelity term with u0 independent of u."""
+    u = RNG.random(shape).astype(np.float32)
+    u0 = RNG.random(shape).astype(np.float32)
+    lam = 0.07
+    ours = oracle.tv_energy_f64(u, u0, lam, etype)
+    ref = oracle.reference_tv_energy(u, u0, lam, etype)
+    bound = (u.size + 8) * 2.0 ** -24
+    assert abs(ours - ref) <= bound * abs(ours), (ours, ref, abs(ours - ref) / abs(ours), bound)
+
+
+@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 6, 4), (6, 11)])
+def test_tv_energy_linear_ramp_closed_form(shape):
+    """u = x + 2 y + 2 z (integers, so every difference is exact): the forward
+    gradient is (1, 2, 2) except that the replicate boundary zeroes the
+    component along each axis at its last index, so |grad u| is sqrt of the
+    sum of the squares of the components still present; counting the points
+    of each boundary class gives the gradient term in closed form."""
+    import itertools
+    dims = tuple(shape)[::-1] + ((1,) if len(shape) == 2 else ())   # (nx, ny, nz)
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = (x + 2 * y + 2 * z).astype(np.float32).reshape(shape)
+    comp = (1.0, 2.0, 2.0)
+    grad = 0.0
+    for last in itertools.product((False, True), repeat=3):   # at the last index along x, y, z?
+        count = 1
+        for n, at_last in zip(dims, last):
+            count *= 1 if at_last else n - 1
+        mag = np.sqrt(sum(c * c for c, at_last in zip(comp, last) if not at_last))
+        grad += count * mag
+    lam = 0.25
+    assert oracle.tv_energy_f64(u, u, lam, 2) == pytest.approx(2 * lam * grad, rel=1e-13)
+    u0 = u + np.float32(0.5)
+    assert oracle.tv_energy_f64(u, u0, lam, 1) == pytest.approx(2 * lam * grad + 0.25 * u.size, rel=1e-13)
 
 
 @pytest.mark.parametrize("f", [F2, F3], ids=["2d", "3d"])
@@ -281,3 +320,24 @@ def test_extent1_axis_equals_duplicated_extent2(shape, axis, method, params):
     if ref is not None:
         y2r, _ = ref.run(method, x2, params)
         assert bits_equal(y2r, y2)
+
+
+@pytest.mark.parametrize("method,params", [
+    ("rof", {"lambda": 0.04, "tau": 0.002, "tol": 1e-30}),
+    ("fgp", {"lambda": 0.04, "tol": 1e-30, "method": 0, "nonneg": 1}),
+    ("tgv", {"lambda": 0.04, "alpha1": 1.0, "alpha0": 2.0, "L2": 12.0, "tol": 1e-30}),
+], ids=["rof", "fgp", "tgv"])
+def test_forced_stop_hook(method, params):
+    """oracle.forced_stop(j) stops the restatement at the check of iteration
+    j: ROF, FGP and TGV then return the iterate after j + 1 iterations, as the
+    fixed-count solve of j + 1 iterations does, and report j; outside the
+    block the rule is the reference's again (a tolerance of 1e-30 never
+    fires)."""
+    j = 10
+    with oracle.forced_stop(j):
+        y, info = oracle.cpu_ref().run(method, F3, dict(params, iters=40))
+    assert int(info[0]) == j
+    z, _ = oracle.cpu_ref().run(method, F3, dict(params, iters=j + 1, tol=0.0))
+    assert np.array_equal(y.view(np.uint32), z.view(np.uint32))
+    _, info2 = oracle.cpu_ref().run(method, F3, dict(params, iters=40))
+    assert int(info2[0]) == 40
