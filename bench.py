@@ -24,13 +24,13 @@ Also reported:
   solved one after another vs in one batched hipGraph launch (FP32 and BF16
   duals); a Poisson request stream (200 req/s, ROF/FGP/PD/TGV, 2D and 3D)
   replayed in real time through the continuous-batching server (p50/p99
-  latency, aggregate voxel-iterations/s) vs the same requests solved plainly.
-  The arrival window defaults to 0.5 s instead of 60 s: the offered load
-  exceeds one device's throughput, so the full window would take many
-  minutes to drain (--workload serve --serve-window 60 runs it).
+  latency, aggregate voxel-iterations/s): the full 60 s stream once with BF16
+  dual storage, and its first 2 s with FP32, BF16 and FP16 storage and
+  solved plainly one after another (the offered load exceeds one device's
+  throughput several times over, so the full stream takes minutes to drain).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu] [--no-batch]
-                       [--no-serve] [--serve-window SECONDS]
+                       [--no-serve] [--serve-window SECONDS] [--serve-full-window SECONDS]
        python bench.py --profile-pass --steps K --warmup 0   (the headline solves only)
        python bench.py --workload {tgv3d,pd3d,sb3d,rof3d,dtv3d,llt3d,ndf3d,d43d,copy,batch,serve} --steps K   (single-workload runs for
        profiler passes; 'copy' copies a 256 MiB buffer K times, the reference kernel of
@@ -412,39 +412,53 @@ def batched_section(D, B, torch):
     }
 
 
-def served_section(D, B, torch, window):
-    """The request stream replayed in real time through the server (FP32 and
-    BF16 duals), and the same requests solved plainly one after another."""
+def served_run(B, reqs, prec, window, label):
+    """One replay of `reqs` in real time through a server with `prec` dual storage."""
     import numpy as np
+    log(f"bench: served stream {label}, {prec}, {len(reqs)} requests")
+    t_start = time.perf_counter()
+
+    def progress(done, total):
+        log(f"bench:   {done}/{total} requests done after {time.perf_counter() - t_start:.0f} s")
+
+    with B.Server(0, hbm_budget=200e9, precision=prec) as srv:
+        r = B.serve_stream(srv, reqs, levels=8, progress=progress)
+    lat = np.array(r["latency_s"])
+    vis = np.array(r["voxels"], dtype=np.float64) * np.array(r["iterations"], dtype=np.float64)
+    vi = float(vis.sum())
+    methods = np.array([sp["method"] for _, sp in reqs])
+    per_method = {}
+    for m in ("rof", "fgp", "pd", "tgv"):
+        sel = methods == m
+        if sel.any():
+            per_method[m] = {"requests": int(sel.sum()), "p50_latency_s": float(np.percentile(lat[sel], 50)),
+                             "p99_latency_s": float(np.percentile(lat[sel], 99)),
+                             "voxel_iters": float(vis[sel].sum())}
+    return {"window_s": window, "requests": len(reqs), "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": r["makespan_s"],
+            "voxel_iters_per_s": vi / r["makespan_s"], "offered_voxel_iters_per_s": vi / window,
+            "mean_jobs_on_device": r["mean_active_per_step"], "server_steps": r["steps"],
+            "per_method": per_method}
+
+
+def served_section(D, B, torch, window, full_window):
+    """The Poisson request stream replayed in real time through the server.
+
+    The configuration's full stream (200 req/s for `full_window` seconds)
+    runs once, with BF16 dual storage (the served form of the metric); the
+    first `window` seconds of the same stream run with FP32, BF16 and FP16
+    storage and plainly (one single solve after another), for the
+    precision comparison and the served-vs-plain ratio."""
     from ccpi.supp.workloads import request_stream, solve_alone
     reqs = request_stream(seconds=window)
-    out = {"workload": f"Poisson 200 req/s for {window:g} s ({len(reqs)} requests), ROF/FGP/PD/TGV, "
-                       "2D 256^2..2048^2 or 3D 64^3..384^3, 100..500 iterations, HBM budget 200 GB"}
+    out = {"workload": "Poisson 200 req/s, ROF/FGP/PD/TGV, 2D 256^2..2048^2 or 3D 64^3..384^3, 100..500 iterations, "
+                       "HBM budget 200 GB",
+           "short_window_s": window}
     # warm-up of code objects and workspaces
     with B.Server(0, hbm_budget=200e9) as srv:
         B.serve_stream(srv, [(0.0, dict(sp, iters=2)) for _, sp in reqs[:8]])
     for prec in ("fp32", "bf16", "fp16"):
-        log(f"bench: served stream {prec}")
-        with B.Server(0, hbm_budget=200e9, precision=prec) as srv:
-            r = B.serve_stream(srv, reqs, levels=8)
-        lat = np.array(r["latency_s"])
-        vis = np.array(r["voxels"], dtype=np.float64) * np.array(r["iterations"], dtype=np.float64)
-        vi = float(vis.sum())
-        methods = np.array([sp["method"] for _, sp in reqs])
-        per_method = {}
-        for m in ("rof", "fgp", "pd", "tgv"):
-            sel = methods == m
-            if sel.any():
-                per_method[m] = {"requests": int(sel.sum()), "p50_latency_s": float(np.percentile(lat[sel], 50)),
-                                 "p99_latency_s": float(np.percentile(lat[sel], 99)),
-                                 "voxel_iters": float(vis[sel].sum())}
-        out[f"served_{prec}"] = {
-            "requests": len(reqs), "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": r["makespan_s"],
-            "voxel_iters_per_s": vi / r["makespan_s"],
-            "offered_voxel_iters_per_s": vi / window,
-            "mean_jobs_on_device": r["mean_active_per_step"], "server_steps": r["steps"],
-            "per_method": per_method}
+        out[f"served_{prec}"] = served_run(B, reqs, prec, window, f"first {window:g} s")
     # plain form: the same requests, one single solve after another (input
     # generation included, as the server generates it at admission)
     log(f"bench: served section, the {len(reqs)} requests solved plainly")
@@ -458,10 +472,16 @@ def served_section(D, B, torch, window):
         del x
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
-    out["plain_fp32"] = {"requests": len(reqs), "seconds": round(tp, 4), "voxel_iters_per_s": vi / tp}
+    out["plain_fp32"] = {"window_s": window, "requests": len(reqs), "seconds": round(tp, 4),
+                         "voxel_iters_per_s": vi / tp}
     for prec in ("fp32", "bf16", "fp16"):
         out[f"served_{prec}"]["vs_plain_voxel_iters"] = (out[f"served_{prec}"]["voxel_iters_per_s"]
                                                           / out["plain_fp32"]["voxel_iters_per_s"])
+    if full_window > 0:
+        full = request_stream(seconds=full_window)
+        r = served_run(B, full, "bf16", full_window, f"full {full_window:g} s")
+        r["vs_plain_voxel_iters"] = r["voxel_iters_per_s"] / out["plain_fp32"]["voxel_iters_per_s"]
+        out[f"served_bf16_full_{full_window:g}s"] = r
     return out
 
 
@@ -535,8 +555,10 @@ def main():
                     help="only the timed headline solves (for rocprofv3 passes)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-window", type=float, default=0.5,
-                    help="seconds of Poisson arrivals replayed by the served section")
+    ap.add_argument("--serve-window", type=float, default=2.0,
+                    help="seconds of Poisson arrivals replayed per precision and plainly by the served section")
+    ap.add_argument("--serve-full-window", type=float, default=60.0,
+                    help="seconds of the full stream replayed once with BF16 duals (0 skips it)")
     ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "sb3d", "rof3d", "dtv3d", "llt3d",
                                                                 "ndf3d", "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
@@ -553,8 +575,8 @@ def main():
         sys.exit(1)
     if args.workload in ("batch", "serve"):
         from ccpi.filters import batch as B
-        res = batched_section(D, B, torch) if args.workload == "batch" else served_section(D, B, torch,
-                                                                                            args.serve_window)
+        res = (batched_section(D, B, torch) if args.workload == "batch"
+               else served_section(D, B, torch, args.serve_window, args.serve_full_window))
         print(json.dumps(res, default=float), flush=True)
         return
     if args.workload != "fgp3d":
@@ -634,7 +656,7 @@ def main():
             torch.cuda.empty_cache()
             D.release_cache()
         if not args.no_serve:
-            extras["served_stream"] = served_section(D, B, torch, args.serve_window)
+            extras["served_stream"] = served_section(D, B, torch, args.serve_window, args.serve_full_window)
             D.release_cache()
 
     line = {

@@ -182,7 +182,9 @@ class Server:
         return t.value
 
     def step(self, levels=1):
-        """Run `levels` iterations of every admitted job; returns (active, pending)."""
+        """Queue `levels` more iterations of every admitted job behind the step
+        already queued, wait for that earlier step and retire what it finished
+        (levels <= 0: wait for everything queued); returns (active, pending)."""
         a, p = ctypes.c_int(), ctypes.c_int()
         if self._lib.cilreg_server_step(self._h, int(levels), ctypes.byref(a), ctypes.byref(p)) != 0:
             raise RuntimeError(f"cilreg_server_step failed: {last_error()}")
@@ -246,7 +248,7 @@ def job_voxels(spec):
     return int(np.prod(shape))
 
 
-def serve_stream(server, requests, levels=8, keep_results=False, clock=None):
+def serve_stream(server, requests, levels=8, keep_results=False, clock=None, progress=None, progress_every=20.0):
     """Replay a timed request stream through `server` in real time.
 
     requests: list of (arrival_seconds, job dict), sorted by arrival.  Each
@@ -255,7 +257,8 @@ def serve_stream(server, requests, levels=8, keep_results=False, clock=None):
     jobs are collected after every step.  Returns a dict with per-request
     latency (completion - arrival, seconds), iterations, voxel counts, the
     makespan and the mean number of jobs on the device per step; with
-    keep_results the outputs too (ticket order = request order)."""
+    keep_results the outputs too (ticket order = request order).  progress:
+    called as progress(done, total) every `progress_every` seconds."""
     import time
     clock = clock or time.perf_counter
     n = len(requests)
@@ -268,7 +271,12 @@ def serve_stream(server, requests, levels=8, keep_results=False, clock=None):
     live = {}
     steps = 0
     occupancy = 0
+    done = 0
+    t_report = t0
     while nxt < n or live:
+        if progress is not None and clock() - t_report >= progress_every:
+            t_report = clock()
+            progress(done, n)
         now = clock() - t0
         while nxt < n and requests[nxt][0] <= now:
             live[server.submit(requests[nxt][1])] = nxt
@@ -284,6 +292,7 @@ def serve_stream(server, requests, levels=8, keep_results=False, clock=None):
             if r is None:
                 continue
             i = live.pop(t)
+            done += 1
             lat[i] = clock() - t0 - requests[i][0]
             iters[i] = int(r[1][0])
             if keep_results:

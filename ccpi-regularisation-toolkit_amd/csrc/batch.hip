@@ -559,9 +559,12 @@ static void make_shapes(int n_shapes, bool is2d, uint64_t seed, std::vector<Shap
     }
 }
 
+// keep: where the shape table lives until its upload has completed (the
+// caller keeps it alive); without it the upload is waited for here.
 static int launch_phantom(float* d_out, int nx, int ny, int nz, int n_shapes, float sigma, uint64_t seed,
-                          Shape* d_shapes, hipStream_t s) {
-    std::vector<Shape> shapes;
+                          Shape* d_shapes, hipStream_t s, std::vector<Shape>* keep = nullptr) {
+    std::vector<Shape> local;
+    std::vector<Shape>& shapes = keep ? *keep : local;
     make_shapes(n_shapes, nz <= 1, seed, shapes);
     if (n_shapes > 0)
         CILREG_CK(hipMemcpyAsync(d_shapes, shapes.data(), n_shapes * sizeof(Shape), hipMemcpyHostToDevice, s));
@@ -570,8 +573,8 @@ static int launch_phantom(float* d_out, int nx, int ny, int nz, int n_shapes, fl
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(phantom_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_out, nx, ny, nz < 1 ? 1 : nz,
                        (const Shape*)d_shapes, n_shapes, sigma, (uint64_t)seed);
-    // the host vector is released on return: wait for the upload
-    CILREG_CK(hipStreamSynchronize(s));
+    // the local shape table is released on return: wait for the upload
+    if (!keep) CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
 
@@ -598,6 +601,7 @@ struct EJob {
     Ctrl* ctrl = nullptr;
     double* partials = nullptr;
     Shape* shapes = nullptr;
+    std::vector<Shape> shapes_host;   // the generated input's shape table until its upload has run
     Geom g{};
     int start_level = 0;
     const float* result = nullptr;   // final buffer once done
@@ -634,8 +638,13 @@ class Engine : public EngineBase {
 public:
     Engine(int dev, double budget, const Opts& o, hipStream_t s) : dev_(dev), budget_(budget), opts_(o), s_(s) {}
     ~Engine() override {
+        (void)hipStreamSynchronize(s_);
+        inflight_.clear();
+        pending_keep_.clear();
         for (auto& g : side_) (void)hipStreamDestroy(g);
         for (auto& e : events_) (void)hipEventDestroy(e);
+        for (auto& e : step_events_) (void)hipEventDestroy(e);
+        for (auto& p : pinned_) (void)hipHostFree(p.first);
         if (mult_) pool_free(mult_);
         for (auto& kv : tables_) if (kv.second.dev) pool_free(kv.second.dev);
         if (fin_dev_) pool_free(fin_dev_);
@@ -677,19 +686,34 @@ public:
         return 0;
     }
 
+    // One server step: admit what fits, queue `levels` more levels of every
+    // active job behind the step already queued, then wait only for that
+    // earlier step and retire the jobs it finished.  The device therefore
+    // always has the newest step queued while the host admits, retires and
+    // returns (admission happens at level boundaries).  levels <= 0 drains:
+    // waits for all queued work and retires what it finished.
     int step(int levels, int* n_active, int* n_pending) override {
-        levels += levels & 1;   // even, so jobs are admitted at even levels (two-level FGP passes)
-        if (admit() != 0) return -1;
-        if (!active_.empty() && levels > 0) {
-            // the job tables depend only on the set of active jobs
-            if (tables_dirty_) {
-                if (build_tables() != 0) return -1;
-                tables_dirty_ = false;
+        if (levels <= 0) {
+            if (drain() != 0) return -1;
+        } else {
+            levels += levels & 1;   // even, so jobs are admitted at even levels (two-level passes)
+            if (admit() != 0) return -1;
+            if (!active_.empty()) {
+                // the job tables depend only on the set of active jobs
+                if (tables_dirty_) {
+                    if (build_tables() != 0) return -1;
+                    tables_dirty_ = false;
+                }
+                if (launch_levels(level_, level_ + levels, false) != 0) return -1;
+                level_ += levels;
+                if (record_step() != 0) return -1;
             }
-            if (launch_levels(level_, level_ + levels, false) != 0) return -1;
-            level_ += levels;
-            CILREG_CK(hipStreamSynchronize(s_));
-            if (retire() != 0) return -1;
+            while (inflight_.size() > 1 || (active_.empty() && !inflight_.empty()))
+                if (complete_oldest() != 0) return -1;
+            if (inflight_.empty() && !pending_keep_.empty()) {
+                CILREG_CK(hipStreamSynchronize(s_));
+                pending_keep_.clear();
+            }
         }
         if (n_active) *n_active = (int)active_.size();
         if (n_pending) *n_pending = (int)pending_.size();
@@ -702,7 +726,7 @@ public:
         EJob& j = *it->second;
         if (j.state == J_FAILED) {
             set_error("%s", j.err[0] ? j.err : "job failed");
-            all_.erase(it);
+            release_job(it);
             return -1;
         }
         if (j.state != J_DONE) return 0;
@@ -712,7 +736,7 @@ public:
             CILREG_CK(hipStreamSynchronize(s_));
         }
         used_ -= (double)j.bytes;
-        all_.erase(it);
+        release_job(it);
         return 1;
     }
 
@@ -723,7 +747,7 @@ public:
     }
 
     // Removes a job in any state; its device memory returns to the pool once
-    // the kernels already queued (which may still use it) have finished.
+    // the work already queued (which may still read it) has finished.
     int cancel(long long ticket) override {
         auto it = all_.find(ticket);
         if (it == all_.end()) { set_error("unknown ticket %lld", ticket); return -1; }
@@ -731,14 +755,14 @@ public:
         if (j->state == J_PENDING) {
             pending_.erase(std::find(pending_.begin(), pending_.end(), j));
         } else {
-            CILREG_CK(hipStreamSynchronize(s_));
+            if (drain() != 0) return -1;   // retire what the queued steps finished first
             if (j->state == J_ACTIVE) {
                 active_.erase(std::find(active_.begin(), active_.end(), j));
                 tables_dirty_ = true;
             }
             if (j->state == J_ACTIVE || j->state == J_DONE) used_ -= (double)j->bytes;
         }
-        all_.erase(it);
+        release_job(it);
         return 0;
     }
 
@@ -772,8 +796,8 @@ public:
                 if (e != hipSuccess) { set_error("batch graph launch failed: %s", hipGetErrorString(e)); return -1; }
             }
             level_ = end + (end & 1);
-            CILREG_CK(hipStreamSynchronize(s_));
-            if (retire() != 0) return -1;
+            if (record_step() != 0) return -1;
+            if (drain() != 0) return -1;
         }
         for (int i = 0; i < n; ++i) {
             EJob& j = *all_[tickets[i]];
@@ -815,6 +839,116 @@ private:
     std::vector<hipStream_t> side_;
     std::vector<hipEvent_t> events_;
 
+    // Steps queued on s_ and not yet retired: each with its completion event,
+    // the level it reaches, the control blocks of its tolerance jobs copied to
+    // pinned memory behind it (by ticket: a job may be polled and another one
+    // take its address while the record waits), and what has to stay alive
+    // until it has run (host staging of table uploads, buffers released while
+    // it was queued).
+    struct StepRec {
+        hipEvent_t ev = nullptr;
+        int level_end = 0;
+        int pinned = -1;                            // snapshot buffer, index into pinned_
+        std::vector<std::pair<long long, int>> snap;   // ticket -> slot of its Ctrl in the snapshot
+        std::vector<std::shared_ptr<void>> keep;
+    };
+    std::deque<StepRec> inflight_;
+    std::vector<std::shared_ptr<void>> pending_keep_;   // released after the next recorded step has run
+    std::vector<hipEvent_t> step_events_;               // free completion events
+    std::vector<std::pair<Ctrl*, int>> pinned_;         // pinned snapshot buffers (pointer, capacity)
+    std::vector<int> pinned_free_;
+
+    void defer(std::shared_ptr<void> p) { pending_keep_.push_back(std::move(p)); }
+    void defer_pool(void* p) {
+        if (p) defer(std::shared_ptr<void>(p, [](void* q) { pool_free(q); }));
+    }
+
+    // Erases a job; its device memory returns to the pool once the work
+    // queued so far (which may still read it) has run.
+    void release_job(std::map<long long, std::unique_ptr<EJob>>::iterator it) {
+        EJob& j = *it->second;
+        if (j.ar) defer(std::shared_ptr<Arena>(std::move(j.ar)));
+        all_.erase(it);
+    }
+
+    hipEvent_t get_event() {
+        if (!step_events_.empty()) {
+            hipEvent_t e = step_events_.back();
+            step_events_.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            set_error("cannot create an event");
+            return nullptr;
+        }
+        return e;
+    }
+
+    int get_pinned(int n) {
+        for (size_t i = 0; i < pinned_free_.size(); ++i) {
+            const int k = pinned_free_[i];
+            if (pinned_[k].second >= n) {
+                pinned_free_.erase(pinned_free_.begin() + i);
+                return k;
+            }
+        }
+        const int cap = std::max(n, 64);
+        Ctrl* h = nullptr;
+        if (hipHostMalloc((void**)&h, cap * sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
+            set_error("pinned snapshot allocation failed");
+            return -1;
+        }
+        pinned_.push_back({h, cap});
+        return (int)pinned_.size() - 1;
+    }
+
+    // Records the step just launched: its completion event and, behind it,
+    // the control blocks of the active tolerance jobs.
+    int record_step() {
+        StepRec r;
+        r.level_end = level_;
+        int ntol = 0;
+        for (EJob* j : active_) ntol += j->pub.epsil != 0.0f;
+        if (ntol) {
+            r.pinned = get_pinned(ntol);
+            if (r.pinned < 0) return -1;
+            Ctrl* h = pinned_[r.pinned].first;
+            int slot = 0;
+            for (EJob* j : active_) {
+                if (j->pub.epsil == 0.0f) continue;
+                CILREG_CK(hipMemcpyAsync(&h[slot], j->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s_));
+                r.snap.push_back({j->ticket, slot++});
+            }
+        }
+        r.ev = get_event();
+        if (!r.ev) return -1;
+        CILREG_CK(hipEventRecord(r.ev, s_));
+        r.keep.swap(pending_keep_);
+        inflight_.push_back(std::move(r));
+        return 0;
+    }
+
+    // Waits for the oldest queued step and retires the jobs it finished.
+    int complete_oldest() {
+        StepRec& r = inflight_.front();
+        CILREG_CK(hipEventSynchronize(r.ev));
+        retire(r.level_end, r.snap, r.pinned >= 0 ? pinned_[r.pinned].first : nullptr);
+        step_events_.push_back(r.ev);
+        if (r.pinned >= 0) pinned_free_.push_back(r.pinned);
+        inflight_.pop_front();
+        return 0;
+    }
+
+    // Waits for all queued work, retiring what it finished.
+    int drain() {
+        while (!inflight_.empty())
+            if (complete_oldest() != 0) return -1;
+        CILREG_CK(hipStreamSynchronize(s_));
+        pending_keep_.clear();
+        return 0;
+    }
+
     // Reserve every buffer of the job (nothing is allocated yet): method state
     // in fld order, control block, per-block partials, and for a generated
     // input the input itself and its shape table.
@@ -851,21 +985,23 @@ private:
 
     int ensure_mult(int iters) {
         if (iters <= mult_cap_) return 0;
-        const int cap = std::max(iters, 2 * mult_cap_);
-        std::vector<T> h(cap);
+        const int cap = std::max(std::max(iters, 2 * mult_cap_), 1024);
+        auto h = std::make_shared<std::vector<T>>(cap);
         T tk = 1, m = 0;
         for (int k = 0; k < cap; ++k) {   // FGP_TV_core.c:138,159 (the single solver's sequence)
-            h[k] = m;
+            (*h)[k] = m;
             const T tkp1 = ((T)1 + dsqrt_host(((T)1 + (T)4 * tk * tk))) * (T)0.5;
             m = (tk - (T)1) / tkp1;
             tk = tkp1;
         }
         T* d = (T*)pool_alloc(cap * sizeof(T));
         if (!d) { set_error("mult table allocation failed"); return -1; }
-        CILREG_CK(hipMemcpy(d, h.data(), cap * sizeof(T), hipMemcpyHostToDevice));
-        if (mult_) pool_free(mult_);
+        CILREG_CK(hipMemcpyAsync(d, h->data(), cap * sizeof(T), hipMemcpyHostToDevice, s_));
+        defer(h);
+        defer_pool(mult_);   // queued steps may still read the old table
         mult_ = d;
         mult_cap_ = cap;
+        tables_dirty_ = true;
         return 0;
     }
 
@@ -884,7 +1020,7 @@ private:
             j.f = (const float*)ar.at(j.off_f);
             j.shapes = (Shape*)ar.at(j.off_sh);
             if (launch_phantom((float*)j.f, j.pub.dimX, j.pub.dimY, j.nz, j.pub.phantom_shapes, j.pub.phantom_sigma,
-                               j.pub.phantom_seed, j.shapes, s_) != 0)
+                               j.pub.phantom_seed, j.shapes, s_, &j.shapes_host) != 0)
                 return -1;
         }
         // initial state, as the single solvers
@@ -1094,27 +1230,33 @@ private:
             t.offsets.push_back(off);
             t.fin_n = (int)fins.size() - t.fin_off;
             t.njobs = (int)t.jobs.size();
+            // the uploads are ordered behind the queued steps, which read the
+            // previous contents; a table that grows gets a new buffer and the
+            // old one is released once they have run
             if (t.host.size() > t.cap) {
-                if (t.dev) pool_free(t.dev);
+                defer_pool(t.dev);
                 t.cap = std::max(t.host.size(), 2 * t.cap);
                 t.dev = pool_alloc(t.cap);
                 if (!t.dev) { set_error("job table allocation failed"); return -1; }
             }
-            if (!t.host.empty())
-                CILREG_CK(hipMemcpyAsync(t.dev, t.host.data(), t.host.size(), hipMemcpyHostToDevice, s_));
+            if (!t.host.empty()) {
+                auto stage = std::make_shared<std::vector<unsigned char>>(t.host);   // lives until the copy ran
+                CILREG_CK(hipMemcpyAsync(t.dev, stage->data(), stage->size(), hipMemcpyHostToDevice, s_));
+                defer(stage);
+            }
         }
         nfin_ = (int)fins.size();
         if (nfin_) {
             if ((size_t)nfin_ * sizeof(FinJob) > fin_cap_) {
-                if (fin_dev_) pool_free(fin_dev_);
+                defer_pool(fin_dev_);
                 fin_cap_ = std::max((size_t)nfin_ * sizeof(FinJob), 2 * fin_cap_);
                 fin_dev_ = (FinJob*)pool_alloc(fin_cap_);
                 if (!fin_dev_) { set_error("finaliser table allocation failed"); return -1; }
             }
-            CILREG_CK(hipMemcpyAsync(fin_dev_, fins.data(), nfin_ * sizeof(FinJob), hipMemcpyHostToDevice, s_));
+            auto stage = std::make_shared<std::vector<FinJob>>(fins);
+            CILREG_CK(hipMemcpyAsync(fin_dev_, stage->data(), nfin_ * sizeof(FinJob), hipMemcpyHostToDevice, s_));
+            defer(stage);
         }
-        // the host tables must outlive the asynchronous copies
-        CILREG_CK(hipStreamSynchronize(s_));
         return 0;
     }
 
@@ -1236,24 +1378,20 @@ private:
         return 0;
     }
 
-    int retire() {
-        // only jobs with a tolerance can stop early: the others finish at
-        // start_level + iter, and their control blocks stay as reset
-        std::vector<Ctrl> c(active_.size());
-        bool copied = false;
-        for (size_t i = 0; i < active_.size(); ++i) {
-            if (active_[i]->pub.epsil == 0.0f) continue;
-            CILREG_CK(hipMemcpyAsync(&c[i], active_[i]->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s_));
-            copied = true;
-        }
-        if (copied) CILREG_CK(hipStreamSynchronize(s_));
+    // Retires the active jobs that a completed step finished: fixed counts
+    // that reached level_done, and tolerance jobs whose control block (in the
+    // step's snapshot h, by ticket) says the rule fired.
+    void retire(int level_done, const std::vector<std::pair<long long, int>>& snap, const Ctrl* h) {
         std::vector<EJob*> still;
-        for (size_t i = 0; i < active_.size(); ++i) {
-            EJob& j = *active_[i];
+        for (EJob* jp : active_) {
+            EJob& j = *jp;
+            Ctrl c{};
+            for (const auto& q : snap)
+                if (q.first == j.ticket) { c = h[q.second]; break; }
             const int iters = j.pub.iter;
-            const bool stopped = c[i].stopped != 0;
-            if (!stopped && level_ < j.start_level + iters) { still.push_back(&j); continue; }
-            const int done = stopped ? c[i].stop_iter : iters;
+            const bool stopped = c.stopped != 0;
+            if (!stopped && level_done < j.start_level + iters) { still.push_back(&j); continue; }
+            const int done = stopped ? c.stop_iter : iters;
             const int last = stopped ? done : iters - 1;
             const int d = j.dim;
             switch (j.pub.method) {
@@ -1279,12 +1417,11 @@ private:
                     break;
             }
             j.info[0] = (float)done;
-            j.info[1] = c[i].re;
+            j.info[1] = c.re;
             j.state = J_DONE;
         }
         if (still.size() != active_.size()) tables_dirty_ = true;
         active_.swap(still);
-        return 0;
     }
 };
 

@@ -215,10 +215,16 @@ CILREG_API int cilreg_phantom_dev(float *d_out, int dimX, int dimY, int dimZ, in
                                   unsigned long long seed, void *stream);
 
 /* Continuous-batching server.  Jobs are admitted while their device memory
- * fits the HBM budget; each step runs `levels` iterations of every admitted
- * job with grouped kernels (jobs at different iteration counts share the same
- * launches), then retires finished jobs.  State stays on the device between
- * steps.  A job's result is bit-identical to solving it alone. */
+ * fits the HBM budget; all admitted jobs advance together through grouped
+ * kernels (jobs at different iteration counts share the same launches) and
+ * their state stays on the device between steps.  cilreg_server_step queues
+ * `levels` more iterations of every admitted job behind the step already
+ * queued, then waits only for that earlier step and retires the jobs it
+ * finished: the device always has the newest step to run while the caller
+ * admits, polls and calls again (a job is reported done one step after its
+ * last iteration was queued).  levels <= 0 waits for all queued work and
+ * retires what it finished.  A job's result is bit-identical to solving it
+ * alone. */
 typedef struct cilreg_server cilreg_server;
 CILREG_API cilreg_server *cilreg_server_create(int gpu_device, double hbm_budget_bytes, const cilreg_opts *opts);
 CILREG_API int cilreg_server_submit(cilreg_server *srv, const cilreg_job *job, long long *ticket);
