@@ -107,6 +107,7 @@ int fgp_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, int method, i
 
     const int nbuf = width == 2 ? 4 : 3;
     Arena ar;
+    if (opts.scratch) ar.use_external(opts.scratch, opts.scratch_bytes);   // the allocation-free entry point
     size_t oP[4][3], oD[2], oCtrl, oPart;
     for (int b = 0; b < nbuf; ++b)
         for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));

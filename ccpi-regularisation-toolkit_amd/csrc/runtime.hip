@@ -212,7 +212,7 @@ ProfileStats& thread_profile() {
 }
 
 Arena::~Arena() {
-    if (base_) pool_free(base_);
+    if (base_ && !ext_) pool_free(base_);
 }
 
 size_t Arena::reserve(size_t bytes) {
@@ -223,6 +223,14 @@ size_t Arena::reserve(size_t bytes) {
 
 int Arena::allocate() {
     if (total_ == 0) total_ = 256;
+    if (ext_) {
+        if (total_ > ext_cap_) {
+            set_error("the solve needs %zu bytes of workspace, above the %zu-byte scratch buffer", total_, ext_cap_);
+            return -1;
+        }
+        base_ = ext_;
+        return 0;
+    }
     base_ = pool_alloc(total_);
     if (!base_) {
         set_error("device allocation of %zu bytes failed", total_);

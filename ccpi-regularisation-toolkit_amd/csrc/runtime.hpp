@@ -62,11 +62,16 @@ public:
     // Reserve bytes (256-byte aligned) before allocate(); returns the offset.
     size_t reserve(size_t bytes);
     int allocate();                       // one hipMalloc for everything reserved
+    // Carve the reservations from caller-owned device memory of `cap` bytes
+    // instead: allocate() then fails if they do not fit and allocates nothing.
+    void use_external(void* base, size_t cap) { ext_ = base; ext_cap_ = cap; }
     void* at(size_t off) const { return (char*)base_ + off; }
     size_t bytes() const { return total_; }
 private:
     void* base_ = nullptr;
     size_t total_ = 0;
+    void* ext_ = nullptr;
+    size_t ext_cap_ = 0;
 };
 
 // Pinned host mirror of the control block (for stop polling); one cached
@@ -193,6 +198,11 @@ struct Opts {
     int stream_launches = 0;  // batch: launch directly instead of one hipGraph
     int depth = 0;         // iterations per pass where supported (FGP, PD, TGV 3D; FGP 2D below 2^20 pixels: 2 by default; 1 = off)
     int autotune = 0;      // choose tile_y / zchunk / depth by timing (tune.hip) when none is given
+    // Allocation-free solves: the workspace is carved from this device memory
+    // (scratch_bytes long) and the stop flag is not polled through pinned
+    // host memory (after a stop the remaining launches return at once).
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
 
 // 2D FGP runs two iterations per pass below this many pixels: there the
@@ -258,11 +268,12 @@ int drive_passes(int iter, int width, float epsil, const Opts& opts, Ctrl* d_ctr
                  double* d_partials1, int nblocks, hipStream_t s, StepFn&& step, int* iters_done, float* re) {
     PinnedCtrl pin;
     const bool use_tol = epsil != 0.0f;
-    if (use_tol && pin.allocate() != 0) return -1;
+    const bool polls = use_tol && !opts.scratch;   // an allocation-free solve does not poll
+    if (polls && pin.allocate() != 0) return -1;
     launch_ctrl_reset(d_ctrl, s);
     hipEvent_t ev[2] = {nullptr, nullptr};
     int flag_slot_pending[2] = {0, 0};
-    if (use_tol) {
+    if (polls) {
         CILREG_CK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         CILREG_CK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     }
@@ -296,7 +307,7 @@ int drive_passes(int iter, int width, float epsil, const Opts& opts, Ctrl* d_ctr
         }
         k += nk;
         since_poll += nk;
-        if (use_tol && since_poll >= opts.poll && k < iter) {
+        if (polls && since_poll >= opts.poll && k < iter) {
             since_poll = 0;
             const int slot = poll_idx & 1;
             if (hipMemcpyAsync(&pin.h[slot].stopped, &d_ctrl->stopped, sizeof(int), hipMemcpyDeviceToHost, s) !=

@@ -136,6 +136,25 @@ CILREG_API int cilreg_dtv_dev(const float *d_input, const float *d_ref, float *d
                    float lambdaPar, int iter, float epsil, float eta, int methodTV, int nonneg, int dimX, int dimY,
                    int dimZ, const cilreg_opts *opts, void *stream);
 
+/* FGP-TV without any device allocation: cilreg_fgp_dev with the same
+ * arguments, results and stopping rule (bit-identical to it), but the
+ * solve's workspace (dual fields, objective buffers, control block, residual
+ * partials) is carved from a fixed device array of
+ * cilreg_noalloc_scratch_bytes() bytes (64 MiB) that each GPU holds from
+ * library load until the process ends; no hipMalloc, hipFree or other
+ * allocating call is made (opts.autotune and opts.profile are ignored, the
+ * stop flag is not polled through pinned memory).  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code that must
+ * not call hipMalloc, serving loops, harnesses that forbid it).  Volumes
+ * whose workspace exceeds the array (FP32 3D: 56 B per voxel, about 1.19 M
+ * voxels; 2D: 40 B per pixel; BF16/FP16 duals: 32 and 24 B) return -1 with
+ * that reason.  Calls on one device share the array and are serialised by
+ * the library; the array keeps the last call's contents. */
+CILREG_API int cilreg_fgp_dev_noalloc(const float *d_input, float *d_output, float *infovector, float lambdaPar,
+                                      int iter, float epsil, int methodTV, int nonneg, int dimX, int dimY, int dimZ,
+                                      const cilreg_opts *opts, void *stream);
+CILREG_API size_t cilreg_noalloc_scratch_bytes(void);
+
 /* FP64 solves (compute and storage in double; dual_precision is ignored). */
 CILREG_API int cilreg_fgp_dev_f64(const double *d_input, double *d_output, double *infovector, double lambdaPar, int iter,
                        double epsil, int methodTV, int nonneg, int dimX, int dimY, int dimZ,

@@ -27,7 +27,7 @@ from conftest import bits_equal, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TGV_ITERS = int(os.environ.get("CILREG_TEST_TGV_ITERS", "100"))
+TGV_ITERS = int(os.environ.get("CILREG_TEST_TGV_ITERS", "500"))
 SERVE_N = 50
 
 
