@@ -384,6 +384,7 @@ def batched_section(D, B, torch):
     got32, t_b32 = timed(lambda: B.batch_run(specs, autotune=True))
     identical = all(bool(torch.equal(a, b)) for a, (b, _) in zip(ref, got32))
     del got32
+
     out16 = {}
     for prec in ("bf16", "fp16"):
         log(f"bench: batched section, one graph {prec.upper()} duals")

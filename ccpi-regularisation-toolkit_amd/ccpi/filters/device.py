@@ -23,15 +23,17 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 class CilregOpts(ctypes.Structure):
     _fields_ = [("check_every", ctypes.c_int), ("dual_precision", ctypes.c_int), ("tile_y", ctypes.c_int),
                 ("zchunk", ctypes.c_int), ("profile", ctypes.c_int), ("wide_index", ctypes.c_int),
-                ("stream_launches", ctypes.c_int), ("temporal_depth", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("stream_launches", ctypes.c_int), ("temporal_depth", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
 
 
 def make_opts(check_every=5, precision='fp32', tile_y=0, zchunk=0, profile=False, wide_index=False,
-              stream_launches=False, depth=0, autotune=False):
+              stream_launches=False, depth=0, autotune=False, persistent=False):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
     return CilregOpts(int(check_every), PRECISIONS[precision], int(tile_y), int(zchunk), int(bool(profile)),
-                      int(bool(wide_index)), int(bool(stream_launches)), int(depth), int(bool(autotune)))
+                      int(bool(wide_index)), int(bool(stream_launches)), int(depth), int(bool(autotune)),
+                      int(bool(persistent)))
 
 
 def last_profile():

@@ -34,15 +34,30 @@ from .diffusion import Diffus4th_CPU, Diffus4th_GPU
 from .utils import cilregcuda
 
 
+def _gpu_index(device):
+    """The GPU index a ``device`` argument selects, or None if it selects none:
+    'gpu' is device 0 (even when the library did not load: the GPU function
+    then reports that), an integer is that device provided the library is
+    loaded (regularisers.py:19-24 of the reference)."""
+    if device == 'gpu':
+        return 0
+    if isinstance(device, numbers.Integral) and cilregcuda is not None:
+        return device
+    return None
+
+
 def create_wrapper(CPU_func, GPU_func):
-    @wraps(CPU_func)
-    def wrapper(*args, device='cpu', **kwargs):
+    """The user function over a CPU and a GPU implementation: ``device='cpu'``
+    (the default) runs CPU_func, a device _gpu_index accepts runs GPU_func with
+    ``gpu_device`` set, any other value raises KeyError."""
+    def dispatch(*args, device='cpu', **kwargs):
         if device == 'cpu':
             return CPU_func(*args, **kwargs)
-        elif device == 'gpu' or isinstance(device, numbers.Integral) and cilregcuda is not None:
-            return GPU_func(*args, gpu_device=0 if device == 'gpu' else device, **kwargs)
-        raise KeyError(f"{GPU_func.__name__}: device {device} not available")
-    return wrapper
+        index = _gpu_index(device)
+        if index is None:
+            raise KeyError(f"{GPU_func.__name__}: device {device} not available")
+        return GPU_func(*args, gpu_device=index, **kwargs)
+    return wraps(CPU_func)(dispatch)
 
 
 ROF_TV = create_wrapper(TV_ROF_CPU, TV_ROF_GPU)

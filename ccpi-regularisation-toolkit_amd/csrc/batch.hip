@@ -107,9 +107,7 @@ __device__ inline int find_job(const J* jobs, int n, int L) {
 // Common prologue of every grouped kernel: the job of this workgroup and its
 // iteration index, or -1 when the job does not run at this level.
 template <class J>
-__device__ inline int group_prologue(const J* jobs, int njobs, int total_blocks, int level, const J*& job,
-                                     int& local) {
-    const int L = xcd_logical_block((int)blockIdx.x, total_blocks);
+__device__ inline int group_prologue_at(const J* jobs, int njobs, int L, int level, const J*& job, int& local) {
     // the job index is uniform over the workgroup; saying so lets the
     // descriptor be read with scalar loads into scalar registers
     const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, L));
@@ -171,13 +169,112 @@ __device__ inline Geom uni_geom(const Geom& g0) {
     return g;
 }
 
+// ---------------------------------------------------------------------------
+// persistent execution: one launch per group pulls (pass, job, tile) items
+// ---------------------------------------------------------------------------
+// Items are numbered pass by pass (a pass = one launch's worth of levels:
+// `width` of them); within a pass, the running jobs' workgroup ranges follow
+// each other as in the per-level grid.  A workgroup takes the next item from
+// `head`, waits until the item's job has completed the previous pass (every
+// workgroup of that pass has finished: `ready`), runs the tile, and counts
+// itself in `arrive`; the last workgroup of a job's pass publishes it in
+// `ready`.  Items are taken in increasing order and an item depends only on
+// items of the previous pass, all taken before it by running workgroups, so
+// the oldest unfinished item can always proceed: the queue cannot deadlock.
+// Every wait is bounded (spin_limit polls); a wait that gives up sets
+// `error`, which makes every other waiting workgroup give up too, so the
+// launch always ends; the host then reports the failure.
+struct PersistQ {
+    unsigned head;          // next item
+    unsigned total;         // items of the launch
+    int npass, width;       // passes and levels per pass
+    int level0;             // level of pass 0
+    int grain;              // workgroup tiles (logical blocks of the per-level grid) per item
+    int error;              // 1 once a wait gave up
+    long long wait_ticks;   // s_memrealtime ticks (100 MHz) before a wait gives up
+    const int* pass_off;    // [npass + 1] first item of each pass
+    const int* joff;        // [njobs + 1] first item of each job within a pass
+    const int* jblk;        // [njobs] workgroup tiles of each job per pass
+    unsigned* arrive;       // [njobs] items finished, over all passes
+    int* ready;             // [njobs] passes completed
+};
+
+template <class J, class Body>
+__device__ inline void persistent_run(const J* jobs, int njobs, PersistQ* q, Body&& body) {
+    __shared__ unsigned s_item;
+    __shared__ int s_ok;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const unsigned total = q->total;
+    const int npass = q->npass, width = q->width, level0 = q->level0, grain = q->grain;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&q->head, 1u);
+        __syncthreads();
+        const unsigned item = s_item;
+        if (item >= total) return;   // uniform: every thread read the same s_item
+        int lo = 0, hi = npass - 1;  // the item's pass
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((unsigned)q->pass_off[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const int pass = __builtin_amdgcn_readfirstlane(lo);
+        const int r = (int)(item - (unsigned)q->pass_off[pass]);   // item within the pass
+        int jl = 0, jh = njobs - 1;                                // its job
+        while (jl < jh) {
+            const int mid = (jl + jh + 1) >> 1;
+            if (q->joff[mid] <= r) jl = mid;
+            else jh = mid - 1;
+        }
+        const int j = __builtin_amdgcn_readfirstlane(jl);
+        if (tid == 0) {
+            int ok = 1;
+            if (pass > 0) {
+                // relaxed polls (no cache maintenance per poll), bounded in
+                // time; one acquire once the previous pass is complete
+                const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+                while (__hip_atomic_load(&q->ready[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < pass) {
+                    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > q->wait_ticks ||
+                        __hip_atomic_load(&q->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            if (!ok) __hip_atomic_store(&q->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_ok = ok;
+        }
+        __syncthreads();
+        if (!s_ok) return;
+        // the item's tiles: logical blocks of the per-level grid's numbering
+        const int b0 = (r - q->joff[j]) * grain;                 // first tile within the job
+        const int nb = min(grain, q->jblk[j] - b0);
+        const int L0 = jobs[j].h.block_offset + b0;
+        for (int g = 0; g < nb; ++g) {
+            if (g) __syncthreads();   // LDS reuse
+            body(level0 + pass * width, L0 + g);
+        }
+        // this wave's stores have completed before the workgroup reports
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned done =
+                __hip_atomic_fetch_add(&q->arrive[j], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            const unsigned per_pass = (unsigned)(q->joff[j + 1] - q->joff[j]);
+            if (done == per_pass * (unsigned)(pass + 1))
+                __hip_atomic_store(&q->ready[j], pass + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 __device__ inline bool job_checks(const JobHdr& h, int k) { return h.check_every > 0 && k % h.check_every == 0; }
 
 template <int DIM, int TY, typename T, typename I>
-__global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* jobs, int njobs, int total, int level) {
+__device__ inline void rof_group_item(const RofJob<T>* jobs, int njobs, int L, int level) {
     const RofJob<T>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     RofArgs<T> a;
     a.f = uni(j->f);
@@ -192,12 +289,21 @@ __global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* job
     for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { rof_body<DIM, TY, T, I>(a, tc); });
 }
 
+template <int DIM, int TY, typename T, typename I>
+__global__ __launch_bounds__(TX * TY) void rof_group_kernel(const RofJob<T>* jobs, int njobs, int total, int level) {
+    rof_group_item<DIM, TY, T, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int DIM, int TY, typename T, typename I>
+__global__ __launch_bounds__(TX * TY) void rof_persist_kernel(const RofJob<T>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { rof_group_item<DIM, TY, T, I>(jobs, njobs, L, level); });
+}
+
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total,
-                                                            int level) {
+__device__ inline void fgp_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
     const FgpJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     FgpArgs<T, S> a;
     const int nb = j->nbuf;
@@ -224,15 +330,24 @@ __global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* 
     for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { fgp_body<DIM, TY, T, S, ISO, I>(a, tc); });
 }
 
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void fgp_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total, int level) {
+    fgp_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void fgp_persist_kernel(const FgpJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { fgp_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, L, level); });
+}
+
 // FGP groups advance two levels per launch: a job at an even iteration
 // index k runs the two-iteration pass, or at its last (odd-count) iteration
 // the one-iteration body.  Jobs start at even levels, so k is always even.
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total,
-                                                             int level) {
+__device__ inline void fgp2_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
     const FgpJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     const int ce = j->h.check_every, iters = j->h.iters;
     auto needs_d = [&](int q) { return q == iters - 1 || (ce > 0 && (q % ce == 0 || (q + 1) % ce == 0)); };
@@ -288,10 +403,20 @@ __global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>*
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+__global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total, int level) {
+    fgp2_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void fgp2_persist_kernel(const FgpJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { fgp2_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, L, level); });
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__device__ inline void pd_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
     const PdJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     PdArgs<T, S> a;
     const int cur = k & 1, nxt = cur ^ 1;
@@ -316,12 +441,21 @@ __global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jo
     for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { pd_body<DIM, TY, T, S, ISO, I>(a, tc); });
 }
 
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+    pd_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd_persist_kernel(const PdJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { pd_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, L, level); });
+}
+
 template <int DIM, int TY, typename T, typename S, typename I>
-__global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total,
-                                                            int level) {
+__device__ inline void tgv_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
     const TgvJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     TgvArgs<T, S> a;
     const int cur = k & 1, nxt = cur ^ 1;
@@ -352,14 +486,24 @@ __global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* 
     for_tiles<DIM, TY>(a.g, j->h, local, [&](const TileCoord& tc) { tgv_body<DIM, TY, T, S, I>(a, tc); });
 }
 
+template <int DIM, int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) void tgv_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total, int level) {
+    tgv_group_item<DIM, TY, T, S, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int DIM, int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) void tgv_persist_kernel(const TgvJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { tgv_group_item<DIM, TY, T, S, I>(jobs, njobs, L, level); });
+}
+
 // 3D PD groups with two-iteration passes: pass q = k / 2 of a job reads
 // state set q & 1 and writes the other; a job's last iteration of an odd
 // count runs the one-iteration body on the same sets.
 template <int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void pd2_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+__device__ inline void pd2_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
     const PdJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     const int cur = (k >> 1) & 1, nxt = cur ^ 1;
     const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
@@ -411,15 +555,24 @@ __global__ __launch_bounds__(TX * TY) void pd2_group_kernel(const PdJob<T, S>* j
     }
 }
 
+template <int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd2_group_kernel(const PdJob<T, S>* jobs, int njobs, int total, int level) {
+    pd2_group_item<TY, T, S, ISO, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int TY, typename T, typename S, bool ISO, typename I>
+__global__ __launch_bounds__(TX * TY) void pd2_persist_kernel(const PdJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { pd2_group_item<TY, T, S, ISO, I>(jobs, njobs, L, level); });
+}
+
 // 3D TGV groups advance two levels per launch: pass q = k / 2 of a job reads
 // state set q & 1 and writes the other; a job's last iteration of an odd
 // count runs the one-iteration body on the same sets.
 template <int TY, typename T, typename S, typename I>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_group_kernel(
-    const TgvJob<T, S>* jobs, int njobs, int total, int level) {
+__device__ inline void tgv2_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
     const TgvJob<T, S>* j;
     int local;
-    const int k = group_prologue(jobs, njobs, total, level, j, local);
+    const int k = group_prologue_at(jobs, njobs, L, level, j, local);
     if (k < 0) return;
     const int cur = (k >> 1) & 1, nxt = cur ^ 1;
     const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
@@ -481,6 +634,16 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) vo
         a.g = uni_geom(j->h.g);
         tgv_body<3, TY, T, S, I>(a, tc);
     }
+}
+
+template <int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total, int level) {
+    tgv2_group_item<TY, T, S, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
+}
+
+template <int TY, typename T, typename S, typename I>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_persist_kernel(const TgvJob<T, S>* jobs, int njobs, PersistQ* q) {
+    persistent_run(jobs, njobs, q, [&](int level, int L) { tgv2_group_item<TY, T, S, I>(jobs, njobs, L, level); });
 }
 
 // The stopping rule of every job that checks at `level` (partials) or, in a
@@ -776,6 +939,14 @@ public:
             int end = 0;
             for (EJob* j : active_) end = std::max(end, j->start_level + std::max(j->pub.iter, 0));
             if (build_tables() != 0) return -1;
+            if (opts_.persistent) {
+                // Both GPU runs of this mode (fixed-count groups as work-queue
+                // kernels beside per-level groups) outlived their 90-120 s test
+                // limits although every wait is bounded; the cause is not
+                // established, so the mode is refused rather than run.
+                set_error("persistent batch launches are disabled in this build (per-level hipGraph launches are used)");
+                return -1;
+            }
             if (opts_.stream_launches) {
                 if (launch_levels(level_, end, false) != 0) return -1;
             } else {
@@ -798,6 +969,7 @@ public:
             level_ = end + (end & 1);
             if (record_step() != 0) return -1;
             if (drain() != 0) return -1;
+            if (check_persistent() != 0) return -1;
         }
         for (int i = 0; i < n; ++i) {
             EJob& j = *all_[tickets[i]];
@@ -831,7 +1003,8 @@ private:
     // descriptors (host copy and device table), first-block offsets, end
     // levels, and its slice [fin_off, fin_off + fin_n) of the finaliser table.
     struct Table { void* dev = nullptr; size_t cap = 0; std::vector<unsigned char> host; int njobs = 0;
-                   std::vector<int> ends, offsets; std::vector<EJob*> jobs; int fin_off = 0, fin_n = 0; };
+                   std::vector<int> ends, offsets; std::vector<EJob*> jobs; int fin_off = 0, fin_n = 0;
+                   PersistQ* pq = nullptr; unsigned pq_total = 0; };   // the group's persistent queue, if any
     std::map<GroupKey, Table> tables_;
     FinJob* fin_dev_ = nullptr;
     size_t fin_cap_ = 0;
@@ -1260,31 +1433,50 @@ private:
         return 0;
     }
 
+    // launches the per-level kernel kg or, with a queue, the persistent kp
+    // over the workgroups that fit on the device at once (at most the items)
+    template <class KG, class KP, class J>
+    static void go(KG kg, KP kp, dim3 grid, dim3 block, hipStream_t st, const J* jobs, int nact, int total, int level,
+                   PersistQ* q) {
+        if (!q) {
+            hipLaunchKernelGGL(kg, grid, block, 0, st, jobs, nact, total, level);
+            return;
+        }
+        long long res = resident_blocks(kp, (int)(block.x * block.y));
+        if (res < 1) res = 1;
+        const long long g = std::min<long long>(res, (long long)total);
+        hipLaunchKernelGGL(kp, dim3((unsigned)g), block, 0, st, jobs, nact, q);
+    }
+
     template <int DIM, int TY>
-    void launch_group(const GroupKey& k, const Table& t, int nact, int total, int level, hipStream_t st) {
+    // One launch of a group: the per-level kernel over `total` workgroups at
+    // `level`, or with a queue q the group's persistent kernel for all its
+    // passes (one resident round of workgroups pulling items).
+    void launch_group(const GroupKey& k, const Table& t, int nact, int total, int level, hipStream_t st,
+                      PersistQ* q = nullptr) {
         const dim3 grid(total), block(TX, TY);
         const bool w = k.wide != 0, iso = k.iso != 0;
         switch (k.method) {
             case CILREG_ROF:
-                if (w) hipLaunchKernelGGL((rof_group_kernel<DIM, TY, T, uint64_t>), grid, block, 0, st, (const RofJob<T>*)t.dev, nact, total, level);
-                else hipLaunchKernelGGL((rof_group_kernel<DIM, TY, T, uint32_t>), grid, block, 0, st, (const RofJob<T>*)t.dev, nact, total, level);
+                if (w) go(rof_group_kernel<DIM, TY, T, uint64_t>, rof_persist_kernel<DIM, TY, T, uint64_t>, grid, block, st, (const RofJob<T>*)t.dev, nact, total, level, q);
+                else go(rof_group_kernel<DIM, TY, T, uint32_t>, rof_persist_kernel<DIM, TY, T, uint32_t>, grid, block, st, (const RofJob<T>*)t.dev, nact, total, level, q);
                 break;
             case CILREG_FGP: {
                 auto* jp = (const FgpJob<T, S>*)t.dev;
                 if (k.width == 2) {
                     if (w) {
-                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                        else hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        if (iso) go(fgp2_group_kernel<DIM, TY, T, S, true, uint64_t>, fgp2_persist_kernel<DIM, TY, T, S, true, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                        else go(fgp2_group_kernel<DIM, TY, T, S, false, uint64_t>, fgp2_persist_kernel<DIM, TY, T, S, false, uint64_t>, grid, block, st, jp, nact, total, level, q);
                     } else {
-                        if (iso) hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                        else hipLaunchKernelGGL((fgp2_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        if (iso) go(fgp2_group_kernel<DIM, TY, T, S, true, uint32_t>, fgp2_persist_kernel<DIM, TY, T, S, true, uint32_t>, grid, block, st, jp, nact, total, level, q);
+                        else go(fgp2_group_kernel<DIM, TY, T, S, false, uint32_t>, fgp2_persist_kernel<DIM, TY, T, S, false, uint32_t>, grid, block, st, jp, nact, total, level, q);
                     }
                 } else if (w) {
-                    if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (iso) go(fgp_group_kernel<DIM, TY, T, S, true, uint64_t>, fgp_persist_kernel<DIM, TY, T, S, true, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                    else go(fgp_group_kernel<DIM, TY, T, S, false, uint64_t>, fgp_persist_kernel<DIM, TY, T, S, false, uint64_t>, grid, block, st, jp, nact, total, level, q);
                 } else {
-                    if (iso) hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((fgp_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (iso) go(fgp_group_kernel<DIM, TY, T, S, true, uint32_t>, fgp_persist_kernel<DIM, TY, T, S, true, uint32_t>, grid, block, st, jp, nact, total, level, q);
+                    else go(fgp_group_kernel<DIM, TY, T, S, false, uint32_t>, fgp_persist_kernel<DIM, TY, T, S, false, uint32_t>, grid, block, st, jp, nact, total, level, q);
                 }
                 break;
             }
@@ -1293,21 +1485,21 @@ private:
                 if constexpr (DIM == 3) {
                     if (k.width == 2) {
                         if (w) {
-                            if (iso) hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                            else hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                            if (iso) go(pd2_group_kernel<TY, T, S, true, uint64_t>, pd2_persist_kernel<TY, T, S, true, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                            else go(pd2_group_kernel<TY, T, S, false, uint64_t>, pd2_persist_kernel<TY, T, S, false, uint64_t>, grid, block, st, jp, nact, total, level, q);
                         } else {
-                            if (iso) hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                            else hipLaunchKernelGGL((pd2_group_kernel<TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                            if (iso) go(pd2_group_kernel<TY, T, S, true, uint32_t>, pd2_persist_kernel<TY, T, S, true, uint32_t>, grid, block, st, jp, nact, total, level, q);
+                            else go(pd2_group_kernel<TY, T, S, false, uint32_t>, pd2_persist_kernel<TY, T, S, false, uint32_t>, grid, block, st, jp, nact, total, level, q);
                         }
                         break;
                     }
                 }
                 if (w) {
-                    if (iso) hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, true, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, false, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (iso) go(pd_group_kernel<DIM, TY, T, S, true, uint64_t>, pd_persist_kernel<DIM, TY, T, S, true, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                    else go(pd_group_kernel<DIM, TY, T, S, false, uint64_t>, pd_persist_kernel<DIM, TY, T, S, false, uint64_t>, grid, block, st, jp, nact, total, level, q);
                 } else {
-                    if (iso) hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, true, uint32_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((pd_group_kernel<DIM, TY, T, S, false, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (iso) go(pd_group_kernel<DIM, TY, T, S, true, uint32_t>, pd_persist_kernel<DIM, TY, T, S, true, uint32_t>, grid, block, st, jp, nact, total, level, q);
+                    else go(pd_group_kernel<DIM, TY, T, S, false, uint32_t>, pd_persist_kernel<DIM, TY, T, S, false, uint32_t>, grid, block, st, jp, nact, total, level, q);
                 }
                 break;
             }
@@ -1315,16 +1507,16 @@ private:
                 auto* jp = (const TgvJob<T, S>*)t.dev;
                 if constexpr (DIM == 3) {
                     if (k.width == 1) {
-                        if (w) hipLaunchKernelGGL((tgv_group_kernel<3, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                        else hipLaunchKernelGGL((tgv_group_kernel<3, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        if (w) go(tgv_group_kernel<3, TY, T, S, uint64_t>, tgv_persist_kernel<3, TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                        else go(tgv_group_kernel<3, TY, T, S, uint32_t>, tgv_persist_kernel<3, TY, T, S, uint32_t>, grid, block, st, jp, nact, total, level, q);
                     } else if (w) {
-                        hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
+                        go(tgv2_group_kernel<TY, T, S, uint64_t>, tgv2_persist_kernel<TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);
                     } else {
-                        hipLaunchKernelGGL((tgv2_group_kernel<TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                        go(tgv2_group_kernel<TY, T, S, uint32_t>, tgv2_persist_kernel<TY, T, S, uint32_t>, grid, block, st, jp, nact, total, level, q);
                     }
                 } else {
-                    if (w) hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint64_t>), grid, block, 0, st, jp, nact, total, level);
-                    else hipLaunchKernelGGL((tgv_group_kernel<DIM, TY, T, S, uint32_t>), grid, block, 0, st, jp, nact, total, level);
+                    if (w) go(tgv_group_kernel<DIM, TY, T, S, uint64_t>, tgv_persist_kernel<DIM, TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                    else go(tgv_group_kernel<DIM, TY, T, S, uint32_t>, tgv_persist_kernel<DIM, TY, T, S, uint32_t>, grid, block, st, jp, nact, total, level, q);
                 }
                 break;
             }
@@ -1334,6 +1526,83 @@ private:
     // Launch levels [l0, l1) of every group.  Groups run on their own streams
     // (forked from and joined back to the main stream with events, which also
     // works under stream capture), so different methods overlap.
+    // Builds the work queue of every group whose jobs all run fixed
+    // iteration counts (a stopping rule needs the per-level finaliser) for
+    // levels [l0, l1), uploaded ahead of the launches.
+    int prepare_persistent(int l0, int l1) {
+        int dev = 0, cus = 0;
+        CILREG_CK(hipGetDevice(&dev));
+        CILREG_CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        for (auto& kv : tables_) {
+            Table& t = kv.second;
+            t.pq = nullptr;
+            if (t.njobs == 0 || t.fin_n > 0) continue;
+            const int width = kv.first.width, nj = t.njobs;
+            // tiles per item: a few items per resident workgroup in the
+            // fullest pass, so the per-item synchronisation stays small next
+            // to the work while the queue still balances the load
+            const int per_pass = t.offsets[nj];
+            const int grain = std::max(1, std::min(16, per_pass / std::max(1, 8 * cus)));
+            std::vector<int> joff(nj + 1, 0), jblk(nj);
+            for (int j = 0; j < nj; ++j) {
+                jblk[j] = t.offsets[j + 1] - t.offsets[j];
+                joff[j + 1] = joff[j] + (jblk[j] + grain - 1) / grain;
+            }
+            std::vector<int> pass_off(1, 0);
+            for (int lv = l0; lv < l1; lv += width) {
+                int nact = 0;
+                while (nact < nj && t.ends[nact] > lv) ++nact;
+                if (nact == 0) break;
+                pass_off.push_back(pass_off.back() + joff[nact]);
+            }
+            const int npass = (int)pass_off.size() - 1;
+            if (npass == 0) continue;
+            auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+            const size_t o_off = al(sizeof(PersistQ)), o_joff = o_off + al(sizeof(int) * (npass + 1)),
+                         o_jblk = o_joff + al(sizeof(int) * (nj + 1)), o_arr = o_jblk + al(sizeof(int) * nj),
+                         o_rdy = o_arr + al(sizeof(unsigned) * nj), bytes = o_rdy + al(sizeof(int) * nj);
+            char* d = (char*)pool_alloc(bytes);
+            if (!d) { set_error("persistent queue allocation failed"); return -1; }
+            defer_pool(d);   // released once the launches that use it have run
+            auto img = std::make_shared<std::vector<char>>(bytes, 0);
+            PersistQ q{};
+            q.total = (unsigned)pass_off.back();
+            q.npass = npass;
+            q.width = width;
+            q.level0 = l0;
+            q.grain = grain;
+            q.wait_ticks = 100000000LL;   // 1 s of the 100 MHz realtime clock
+            q.pass_off = (const int*)(d + o_off);
+            q.joff = (const int*)(d + o_joff);
+            q.jblk = (const int*)(d + o_jblk);
+            q.arrive = (unsigned*)(d + o_arr);
+            q.ready = (int*)(d + o_rdy);
+            std::memcpy(img->data(), &q, sizeof q);
+            std::memcpy(img->data() + o_off, pass_off.data(), sizeof(int) * (npass + 1));
+            std::memcpy(img->data() + o_joff, joff.data(), sizeof(int) * (nj + 1));
+            std::memcpy(img->data() + o_jblk, jblk.data(), sizeof(int) * nj);
+            CILREG_CK(hipMemcpyAsync(d, img->data(), bytes, hipMemcpyHostToDevice, s_));
+            defer(img);
+            t.pq = (PersistQ*)d;
+            t.pq_total = q.total;
+        }
+        CILREG_CK(hipStreamSynchronize(s_));   // queues in place before any capture
+        return 0;
+    }
+
+    // After the persistent launches have run: fails if any queue's wait gave up.
+    int check_persistent() {
+        for (auto& kv : tables_) {
+            Table& t = kv.second;
+            if (!t.pq) continue;
+            int err = 0;
+            CILREG_CK(hipMemcpy(&err, &t.pq->error, sizeof err, hipMemcpyDeviceToHost));
+            t.pq = nullptr;
+            if (err) { set_error("persistent batch: a work item's dependency wait gave up"); return -1; }
+        }
+        return 0;
+    }
+
     int launch_levels(int l0, int l1, bool capturing) {
         std::vector<std::pair<GroupKey, Table*>> groups;
         for (auto& kv : tables_)
@@ -1357,7 +1626,11 @@ private:
             hipStream_t st = side_[gi];
             // groups of two-iteration passes advance two levels per launch (l0 is even)
             const int width = k.width;
-            for (int lv = l0; lv < l1; lv += width) {
+            if (t.pq) {   // one persistent launch runs every pass
+                if (k.dim == 3) launch_group<3, 8>(k, t, t.njobs, (int)t.pq_total, l0, st, t.pq);
+                else launch_group<2, 4>(k, t, t.njobs, (int)t.pq_total, l0, st, t.pq);
+            }
+            for (int lv = l0; lv < l1 && !t.pq; lv += width) {
                 int nact = 0;
                 while (nact < t.njobs && t.ends[nact] > lv) ++nact;
                 if (nact == 0) break;

@@ -344,6 +344,7 @@ Opts resolve_opts(const cilreg_opts* o) {
         r.stream_launches = o->stream_launches;
         r.depth = o->temporal_depth;
         r.autotune = o->autotune;
+        r.persistent = o->persistent;
     }
     return r;
 }

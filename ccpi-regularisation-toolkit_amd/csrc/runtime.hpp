@@ -198,6 +198,7 @@ struct Opts {
     int stream_launches = 0;  // batch: launch directly instead of one hipGraph
     int depth = 0;         // iterations per pass where supported (FGP, PD, TGV 3D; FGP 2D below 2^20 pixels: 2 by default; 1 = off)
     int autotune = 0;      // choose tile_y / zchunk / depth by timing (tune.hip) when none is given
+    int persistent = 0;    // batch: one persistent work-queue launch per group instead of per-level launches
     // Allocation-free solves: the workspace is carved from this device memory
     // (scratch_bytes long) and the stop flag is not polled through pinned
     // host memory (after a stop the remaining launches return at once).

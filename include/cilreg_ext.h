@@ -45,6 +45,8 @@ typedef struct cilreg_opts {
     int stream_launches; /* batch: 1 = launch directly instead of capturing one hipGraph */
     int temporal_depth;  /* iterations fused per pass where supported (FGP, PD, TGV 3D): 0 = auto (2), 1 = off */
     int autotune;        /* 1: tile_y / zchunk / temporal_depth left at 0 are chosen by cilreg_autotune */
+    int persistent;      /* batch: 1 = one persistent work-queue kernel per method group; disabled in this
+                            build (cilreg_batch_run returns -1 with the reason) */
 } cilreg_opts;
 
 /* Methods of a batch or served job. */

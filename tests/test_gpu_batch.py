@@ -72,20 +72,45 @@ def _assert_same(method, shape, got, ref):
     assert bool((a == b).all()), f"{tag}: {int((a != b).sum())} differing elements"
 
 
+PERSISTENT_OFF = ("the persistent work-queue batch mode outlived its test limit on the GPU twice (cause not "
+                  "established); the library refuses it and it stays off the GPU")
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-@pytest.mark.parametrize("graph", [True, False], ids=["graph", "stream"])
-def test_batch_matches_single_solves(precision, graph):
+@pytest.mark.parametrize("mode", ["graph", "stream",
+                                  pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
+def test_batch_matches_single_solves(precision, mode):
+    """In persistent mode the groups whose jobs all run fixed counts (3D ROF,
+    3D PD, 2D TGV here) run as one work-queue kernel each and the others keep
+    per-level launches, in the same batch."""
     from ccpi.filters import batch as B
     mix = _mix()
     xs = _inputs(mix)
     specs = [_spec(m, x, p) for (m, _, p), x in zip(mix, xs)]
-    print(f"batch of {len(specs)} jobs, {precision}, {'graph' if graph else 'stream'} launches", flush=True)
-    got = B.batch_run(specs, precision=precision, stream_launches=not graph)
+    print(f"batch of {len(specs)} jobs, {precision}, {mode} launches", flush=True)
+    got = B.batch_run(specs, precision=precision, stream_launches=mode == "stream", persistent=mode == "persistent")
     for (m, shape, p), x, g in zip(mix, xs, got):
         _assert_same(m, shape, g, _single(m, x, p, precision=precision))
     # at least one tolerance job stopped early, so the stop path is covered
     stopped = [int(g[1][0]) < p["iters"] for (m, _, p), g in zip(mix, got) if p["tol"] > 0]
     assert any(stopped), [g[1] for g in got]
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_persistent_batch_fixed_counts(precision):
+    """Every group persistent (all jobs fixed counts): two-iteration passes
+    with odd tails (3D FGP, PD, TGV), one-iteration groups (ROF, 2D), jobs of
+    different lengths sharing a queue; each result bit-identical to its single
+    solve, repeated to show the queues reset."""
+    from ccpi.filters import batch as B
+    mix = [(m, shape, dict(p, tol=0.0)) for m, shape, p in _mix()]
+    xs = _inputs(mix, seed0=300)
+    specs = [_spec(m, x, p) for (m, _, p), x in zip(mix, xs)]
+    for _ in range(2):
+        got = B.batch_run(specs, precision=precision, persistent=True)
+        for (m, shape, p), x, g in zip(mix, xs, got):
+            _assert_same(m, shape, g, _single(m, x, p, precision=precision))
 
 
 def test_batch_generated_inputs_match_phantom_api():
@@ -245,3 +270,11 @@ def test_phantom_matches_restatement(shape):
     assert np.allclose(got, ref, rtol=0, atol=1e-6), np.abs(got - ref).max()
     again = B.phantom(shape, 4, 0.1, 1234).cpu().numpy()
     assert np.array_equal(got, again)
+
+
+def test_persistent_mode_is_refused():
+    """The disabled mode returns -1 with its reason before any launch."""
+    from ccpi.filters import batch as B
+    x = B.phantom((8, 20, 70), 3, 0.05, 1)
+    with pytest.raises(RuntimeError, match="persistent batch launches are disabled"):
+        B.batch_run([{"method": "rof", "input": x, "iters": 4, "lambda": 0.02, "tau": 0.001}], persistent=True)
