@@ -401,8 +401,22 @@ def batched_section(D, B, torch):
         out16[prec] = {"seconds": round(t16, 4), "voxel_iters_per_s": total_vi / t16, "speedup_vs_plain": t_plain / t16,
                        "fgp_rel_l2_vs_fp64_max_of_4": max(errs) if errs else None}
     del ref
+    # the same split by method: what batching and 16-bit duals can do for
+    # each half, and the bound the ROF half sets on the whole batch (ROF has
+    # no dual field, so its batched time is a floor for any dual storage)
+    split = {}
+    for m in ("fgp", "rof"):
+        sub = [sp for sp in specs if sp["method"] == m]
+        vi = sum(sp["input"].numel() * sp["iters"] for sp in sub)
+        _, tp = timed(lambda: plain(sub, "fp32"))
+        _, tb = timed(lambda: B.batch_run(sub, autotune=True))
+        _, tb16 = timed(lambda: B.batch_run(sub, precision="bf16", autotune=True))
+        split[m] = {"voxel_iters": vi, "plain_fp32_s": round(tp, 4), "batched_fp32_s": round(tb, 4),
+                    "batched_bf16_s": round(tb16, 4), "speedup_fp32": tp / tb, "speedup_bf16": tp / tb16}
+    split["ceiling_speedup_if_fgp_took_no_time"] = t_plain / split["rof"]["batched_bf16_s"]
     return {
         "workload": "64 volumes, extents U{64..320}^3, half FGP-TV 300 iterations, half ROF-TV 1000 iterations",
+        "per_method": split,
         "voxel_iters": total_vi,
         "autotune_seconds": round(t_tune, 3),
         "plain_fp32": {"seconds": round(t_plain, 4), "voxel_iters_per_s": total_vi / t_plain},
@@ -442,7 +456,7 @@ def served_run(B, reqs, prec, window, label):
             "per_method": per_method}
 
 
-def served_section(D, B, torch, window, full_window):
+def served_section(D, B, torch, window, full_window, budget_s=420.0):
     """The Poisson request stream replayed in real time through the server.
 
     The configuration's full stream (200 req/s for `full_window` seconds)
@@ -479,10 +493,22 @@ def served_section(D, B, torch, window, full_window):
         out[f"served_{prec}"]["vs_plain_voxel_iters"] = (out[f"served_{prec}"]["voxel_iters_per_s"]
                                                           / out["plain_fp32"]["voxel_iters_per_s"])
     if full_window > 0:
+        import numpy as np
         full = request_stream(seconds=full_window)
-        r = served_run(B, full, "bf16", full_window, f"full {full_window:g} s")
-        r["vs_plain_voxel_iters"] = r["voxel_iters_per_s"] / out["plain_fp32"]["voxel_iters_per_s"]
-        out[f"served_bf16_full_{full_window:g}s"] = r
+        key = f"served_bf16_full_{full_window:g}s"
+        # predicted from the short window's served rate; a run that would not
+        # drain within the budget is skipped and says so
+        vi_full = float(sum(np.prod(sp["shape"]) * sp["iters"] for _, sp in full))
+        predicted = vi_full / out["served_bf16"]["voxel_iters_per_s"]
+        if predicted > budget_s:
+            out[key] = {"skipped": f"predicted {predicted:.0f} s to drain, above the {budget_s:g} s budget; "
+                                   f"bench.py --workload serve --serve-full-window {full_window:g} "
+                                   "--serve-budget 1e9 runs it"}
+        else:
+            r = served_run(B, full, "bf16", full_window, f"full {full_window:g} s")
+            r["vs_plain_voxel_iters"] = r["voxel_iters_per_s"] / out["plain_fp32"]["voxel_iters_per_s"]
+            r["predicted_drain_s"] = predicted
+            out[key] = r
     return out
 
 
@@ -560,6 +586,8 @@ def main():
                     help="seconds of Poisson arrivals replayed per precision and plainly by the served section")
     ap.add_argument("--serve-full-window", type=float, default=60.0,
                     help="seconds of the full stream replayed once with BF16 duals (0 skips it)")
+    ap.add_argument("--serve-budget", type=float, default=420.0,
+                    help="skip the full stream when its predicted drain time exceeds this many seconds")
     ap.add_argument("--workload", default="fgp3d", choices=["fgp3d", "tgv3d", "pd3d", "sb3d", "rof3d", "dtv3d", "llt3d",
                                                                 "ndf3d", "d43d", "copy", "batch", "serve"])
     args = ap.parse_args()
@@ -577,7 +605,7 @@ def main():
     if args.workload in ("batch", "serve"):
         from ccpi.filters import batch as B
         res = (batched_section(D, B, torch) if args.workload == "batch"
-               else served_section(D, B, torch, args.serve_window, args.serve_full_window))
+               else served_section(D, B, torch, args.serve_window, args.serve_full_window, args.serve_budget))
         print(json.dumps(res, default=float), flush=True)
         return
     if args.workload != "fgp3d":
@@ -657,7 +685,8 @@ def main():
             torch.cuda.empty_cache()
             D.release_cache()
         if not args.no_serve:
-            extras["served_stream"] = served_section(D, B, torch, args.serve_window, args.serve_full_window)
+            extras["served_stream"] = served_section(D, B, torch, args.serve_window, args.serve_full_window,
+                                                     args.serve_budget)
             D.release_cache()
 
     line = {

@@ -200,7 +200,7 @@ struct PersistQ {
 };
 
 template <class J, class Body>
-__device__ inline void persistent_run(const J* jobs, int njobs, PersistQ* q, Body&& body) {
+__device__ __attribute__((always_inline)) inline void persistent_run(const J* jobs, int njobs, PersistQ* q, Body&& body) {
     __shared__ unsigned s_item;
     __shared__ int s_ok;
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -271,7 +271,7 @@ __device__ inline void persistent_run(const J* jobs, int njobs, PersistQ* q, Bod
 __device__ inline bool job_checks(const JobHdr& h, int k) { return h.check_every > 0 && k % h.check_every == 0; }
 
 template <int DIM, int TY, typename T, typename I>
-__device__ inline void rof_group_item(const RofJob<T>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void rof_group_item(const RofJob<T>* jobs, int njobs, int L, int level) {
     const RofJob<T>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(TX * TY) void rof_persist_kernel(const RofJob<T>* j
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__device__ inline void fgp_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void fgp_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
     const FgpJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(TX * TY) void fgp_persist_kernel(const FgpJob<T, S>
 // index k runs the two-iteration pass, or at its last (odd-count) iteration
 // the one-iteration body.  Jobs start at even levels, so k is always even.
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__device__ inline void fgp2_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void fgp2_group_item(const FgpJob<T, S>* jobs, int njobs, int L, int level) {
     const FgpJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(TX * TY) void fgp2_persist_kernel(const FgpJob<T, S
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__device__ inline void pd_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void pd_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
     const PdJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(TX * TY) void pd_persist_kernel(const PdJob<T, S>* 
 }
 
 template <int DIM, int TY, typename T, typename S, typename I>
-__device__ inline void tgv_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void tgv_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
     const TgvJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(TX * TY) void tgv_persist_kernel(const TgvJob<T, S>
 // state set q & 1 and writes the other; a job's last iteration of an odd
 // count runs the one-iteration body on the same sets.
 template <int TY, typename T, typename S, bool ISO, typename I>
-__device__ inline void pd2_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void pd2_group_item(const PdJob<T, S>* jobs, int njobs, int L, int level) {
     const PdJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(TX * TY) void pd2_persist_kernel(const PdJob<T, S>*
 // state set q & 1 and writes the other; a job's last iteration of an odd
 // count runs the one-iteration body on the same sets.
 template <int TY, typename T, typename S, typename I>
-__device__ inline void tgv2_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
+__device__ __attribute__((always_inline)) inline void tgv2_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
     const TgvJob<T, S>* j;
     int local;
     const int k = group_prologue_at(jobs, njobs, L, level, j, local);
