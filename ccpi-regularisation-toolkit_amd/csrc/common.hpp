@@ -235,13 +235,35 @@ __device__ inline void block_sum2(double& a, double& b, double* lds) {
 __device__ inline bool solve_stopped(const Ctrl* c) { return c != nullptr && c->stopped != 0; }
 
 // ---------------------------------------------------------------------------
+// correctly rounded single-precision square root
+// ---------------------------------------------------------------------------
+// For x in [2^-96, FLT_MAX], v_sqrt_f32 is within one ulp of sqrt(x) and the
+// neighbours r -/+ 1 ulp of its result r are normal and finite; the rounded
+// root is r - 1 ulp when x - (r - 1 ulp) r <= 0, r + 1 ulp when
+// x - (r + 1 ulp) r > 0, and r otherwise (both residuals by fma, exact).
+// This is the selection the compiler's IEEE expansion of sqrtf makes after
+// it has scaled tiny arguments and before it patches zeros and infinities,
+// so on that interval it returns sqrtf's bits (checked over every float of
+// it by cilreg_check_sqrt_rn).  A wave with any lane outside the interval
+// (zero, denormal, negative, infinite, NaN) calls sqrtf.
+__device__ inline float sqrt_rn(float x) {
+    const bool fast = x >= 0x1p-96f && x <= 0x1.fffffep127f;   // false for NaN
+    if (__builtin_amdgcn_ballot_w64(!fast)) return sqrtf(x);
+    const float r = __builtin_amdgcn_sqrtf(x);
+    const float rm = __uint_as_float(__float_as_uint(r) - 1u), rp = __uint_as_float(__float_as_uint(r) + 1u);
+    float y = __builtin_fmaf(-rm, r, x) <= 0.f ? rm : r;
+    y = __builtin_fmaf(-rp, r, x) > 0.f ? rp : y;
+    return y;
+}
+
+// ---------------------------------------------------------------------------
 // exact restatements of the reference's mixed-precision expressions
 // ---------------------------------------------------------------------------
 template <typename T> struct Mixed;
 template <> struct Mixed<float> {
     // sqrtf((float)((double)s + eps)): ROF_TV_core.c:160,199,223,262 (eps 1e-8;
     // LLT_ROF_core.c:256,296,320,359 with 1e-12)
-    __device__ static inline float rof_sqrt_f(float s, double eps = 1.0e-8) { return sqrtf((float)((double)s + eps)); }
+    __device__ static inline float rof_sqrt_f(float s, double eps = 1.0e-8) { return sqrt_rn((float)((double)s + eps)); }
     // (float)sqrt((double)s + eps): the 3D D1, ROF_TV_core.c:136, LLT_ROF_core.c:232
     __device__ static inline float rof_sqrt_d(float s, double eps = 1.0e-8) { return (float)sqrt((double)s + eps); }
     // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
@@ -315,7 +337,7 @@ template <> struct Mixed<double> {
 };
 
 template <typename T> __device__ inline T dsqrt(T x);
-template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ inline float dsqrt<float>(float x) { return sqrt_rn(x); }
 template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
 
 }  // namespace cilreg

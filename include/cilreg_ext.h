@@ -180,6 +180,12 @@ CILREG_API int cilreg_tv_energy_dev(const float *d_U, const float *d_U0, double 
 CILREG_API int cilreg_tv_energy_dev_f64(const double *d_U, const double *d_U0, double lambda, int type, int dimX,
                                         int dimY, int dimZ, double *energy, void *stream);
 
+/* Self-check of the kernels' correctly rounded float square root on the
+ * current device: compares it with sqrtf on every 32-bit pattern read as a
+ * float and returns the number of differing results (0 expected) and of
+ * values checked (2^32). */
+CILREG_API int cilreg_check_sqrt_rn(unsigned long long *mismatches, unsigned long long *checked);
+
 /* Achievable HBM bandwidth of the current device, from streaming kernels
  * over arrays of `bytes` each (>= 1 MiB), the mean of `reps` timed launches:
  * gbps[0] copy (1 read + 1 write), gbps[1] read, gbps[2] write, gbps[3]
