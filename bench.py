@@ -99,12 +99,19 @@ def cpu_baseline(x_np, iters):
         lib = oracle.cpu_ref()
         kind = "port"
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), None)
+    except OSError:
+        pass
     p = FGP_PARAMS
     t0 = time.perf_counter()
     lib.fgp(x_np, p["lam"], iters, 0.0, p["method"], p["nonneg"])
     dt = time.perf_counter() - t0
     vi = x_np.size * iters / dt
-    return {"value": vi, "unit": "voxel-iters/s", "cores": threads, "kind": kind,
+    return {"value": vi, "unit": "voxel-iters/s", "cores": threads, "kind": kind, "cpu_model": model,
+            "host_logical_cpus": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "unset"),
             "sample": f"FGP-TV 3D 256^3, {iters} of 400 iterations (fixed count), {dt:.1f} s",
             "seconds": round(dt, 2)}
 

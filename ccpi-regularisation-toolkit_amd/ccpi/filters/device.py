@@ -106,6 +106,7 @@ _SIGS = {
     "cilreg_fgp_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
     "cilreg_pd_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _f, _i, _i, _i, _i, _i, _vp, _i, _vp, _vp],
     "cilreg_tgv_dev_ws": [_vp, _vp, _vp, _f, _f, _f, _i, _f, _f, _i, _i, _i, _vp, _i, _vp, _vp],
+    "cilreg_sb_dev_ws": [_vp, _vp, _vp, _f, _i, _f, _i, _i, _i, _i, _vp, _i, _vp, _vp],
 }
 
 
@@ -262,7 +263,7 @@ def tv_energy(u, u0, lam, type=1):
 # ---------------------------------------------------------------------------
 # warm start (include/cilreg_ext.h: cilreg_*_dev_ws)
 # ---------------------------------------------------------------------------
-STATE_CHANNELS = {"fgp": 6, "pd": 4, "tgv": 13}
+STATE_CHANNELS = {"fgp": 6, "pd": 4, "tgv": 13, "sb": 7}
 
 
 def new_state(method, x):
@@ -332,6 +333,22 @@ def tgv_warm(x, lam, alpha1, alpha0, iters, state, warm=True, L2=12.0, tol=0.0, 
                ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
     if rc != 0:
         raise RuntimeError(f"cilreg_tgv_dev_ws failed: {last_error()}")
+    return out, info
+
+
+def sb_warm(x, mu, iters, state, warm=True, tol=0.0, method=0, out=None, **kw):
+    """SB starting from (warm) and returning the iterate state (U, D, B) in `state`."""
+    import torch
+    out = _prep(x, out, torch.float32)
+    _check_state(state, "sb", x)
+    info = np.zeros(2, dtype=np.float32)
+    f = _fn("cilreg_sb_dev_ws")
+    with torch.cuda.device(x.device):
+        rc = f(x.data_ptr(), out.data_ptr(), info.ctypes.data, mu, iters, tol, method, *_dims(x), state.data_ptr(),
+               int(bool(warm)), ctypes.byref(make_opts(**kw)),
+               ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"cilreg_sb_dev_ws failed: {last_error()}")
     return out, info
 
 

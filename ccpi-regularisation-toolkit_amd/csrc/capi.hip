@@ -491,6 +491,17 @@ int cilreg_pd_dev_ws(const float* d_input, float* d_output, float* infovector, f
     DEV_EPILOGUE()
 }
 
+int cilreg_sb_dev_ws(const float* d_input, float* d_output, float* infovector, float mu, int iter, float epsil,
+                     int methodTV, int dimX, int dimY, int dimZ, float* d_state, int warm, const cilreg_opts* opts,
+                     void* stream) {
+    if (!d_state || iter < 1) { set_error("warm start needs d_state and iter >= 1"); return -1; }
+    DEV_PROLOGUE(dimX, dimY, dimZ, -1, 1)
+    const int nz = dimZ <= 1 ? 1 : dimZ;
+    WarmState ws{d_state, nullptr, warm != 0};
+    rc = sb_solve<float, float>(d_input, d_output, mu, iter, epsil, methodTV, dimX, dimY, nz, o, s, &done, &re, &ws);
+    DEV_EPILOGUE()
+}
+
 int cilreg_tgv_dev_ws(const float* d_input, float* d_output, float* infovector, float lambda, float alpha1,
                       float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ, float* d_state,
                       int warm, const cilreg_opts* opts, void* stream) {

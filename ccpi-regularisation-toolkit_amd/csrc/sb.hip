@@ -18,6 +18,7 @@
 // sweep 1 reads U0, f, D(3), B(3) and writes U1 (36 B); kernel 2 reads U1, f,
 // D, B and writes U2, D, B (60 B).
 #include "solvers.hpp"
+#include "state.hpp"
 
 namespace cilreg {
 
@@ -469,7 +470,7 @@ static const void* sb_fused_ptr(bool aniso, bool small) {
 
 template <typename T, typename S>
 int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int nx, int ny, int nz, const Opts& opts,
-             hipStream_t s, int* iters_done, float* re) {
+             hipStream_t s, int* iters_done, float* re, const WarmState* ws) {
     const int dim = nz <= 1 ? 2 : 3;
     if (dim == 2) nz = 1;
     const size_t n = (size_t)nx * ny * nz;
@@ -510,10 +511,21 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
     oCtrl = ar.reserve(sizeof(Ctrl));
     oPart = ar.reserve((size_t)g.nblocks * 2 * sizeof(double));
     if (ar.allocate() != 0) return -1;
-    launch_copy(ar.at(oU0), d_f, n * sizeof(T), s);
-    for (int c = 0; c < dim; ++c) {
-        launch_zero(ar.at(oD[0][c]), n * sizeof(T), s);
-        launch_zero(ar.at(oB[0][c]), n * sizeof(T), s);
+    // an iteration reads U (buffer U0 at k = 0), D and B (set 0) and nothing
+    // older, so (U, D, B) is the whole state: U = f, D = B = 0 cold
+    // (SB_TV_core.c:58-60), else the caller's channels U, D1..D3, B1..B3
+    if (ws && ws->warm) {
+        state_in<T>((T*)ar.at(oU0), ws->d_state, n, s);
+        for (int c = 0; c < dim; ++c) {
+            state_in<T>((T*)ar.at(oD[0][c]), ws->d_state + (size_t)(1 + c) * n, n, s);
+            state_in<T>((T*)ar.at(oB[0][c]), ws->d_state + (size_t)(4 + c) * n, n, s);
+        }
+    } else {
+        launch_copy(ar.at(oU0), d_f, n * sizeof(T), s);
+        for (int c = 0; c < dim; ++c) {
+            launch_zero(ar.at(oD[0][c]), n * sizeof(T), s);
+            launch_zero(ar.at(oB[0][c]), n * sizeof(T), s);
+        }
     }
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
     double* partials = (double*)ar.at(oPart);
@@ -593,11 +605,19 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
     const int last = (done < iter) ? done : iter - 1;
     const size_t oRes = (fused && ((last + 1) & 1)) ? oU1 : oU0;
     CILREG_CK(hipMemcpyAsync(d_out, ar.at(oRes), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if (ws) {   // the state after the executed iterations: D and B in set (last + 1) mod 2
+        const int set = (last + 1) & 1;
+        state_out<T>(ws->d_state, (const T*)ar.at(oRes), n, s);
+        for (int c = 0; c < dim; ++c) {
+            state_out<T>(ws->d_state + (size_t)(1 + c) * n, (const T*)ar.at(oD[set][c]), n, s);
+            state_out<T>(ws->d_state + (size_t)(4 + c) * n, (const T*)ar.at(oB[set][c]), n, s);
+        }
+    }
     CILREG_CK(hipStreamSynchronize(s));
     return 0;
 }
 
 template int sb_solve<float, float>(const float*, float*, float, int, float, int, int, int, int, const Opts&,
-                                    hipStream_t, int*, float*);
+                                    hipStream_t, int*, float*, const WarmState*);
 
 }  // namespace cilreg

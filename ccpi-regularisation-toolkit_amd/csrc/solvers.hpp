@@ -25,7 +25,7 @@ int pd_solve(const T* d_f, T* d_out, T lambda, int iter, T epsil, T lipschitz, i
              int nz, const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws = nullptr);
 template <typename T, typename S>
 int sb_solve(const T* d_f, T* d_out, T mu, int iter, T epsil, int method, int nx, int ny, int nz, const Opts& opts,
-             hipStream_t s, int* iters_done, float* re);
+             hipStream_t s, int* iters_done, float* re, const WarmState* ws = nullptr);
 template <typename T, typename S>
 int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T L2, T epsil, int nx, int ny, int nz,
               const Opts& opts, hipStream_t s, int* iters_done, float* re, const WarmState* ws = nullptr);

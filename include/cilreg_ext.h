@@ -199,7 +199,9 @@ CILREG_API int cilreg_stream_bench(size_t bytes, int reps, double *gbps);
  *        t_state (host float[2]) = {t_k, multiplier of the next iteration};
  *   PD:  u_bar, P1, P2, P3 (2D: 0, 1, 2);
  *   TGV: u_bar, V1..V3, P1..P3, Q11, Q22, Q33, Q12, Q13, Q23
- *        (2D: u_bar, V1, V2, P1, P2 in channels 0, 1, 2, 4, 5 and Q11, Q22, Q12 in 7, 8, 9).
+ *        (2D: u_bar, V1, V2, P1, P2 in channels 0, 1, 2, 4, 5 and Q11, Q22, Q12 in 7, 8, 9);
+ *   SB:  U, D1..D3, B1..B3 (the split variables and Bregman terms of
+ *        SB_TV_core.c:61-133; 2D: channels 0, 1, 2, 4, 5).
  * warm = 1 starts from the state (and t_state); warm = 0 starts cold like
  * the plain solve.  On return the state is that after the executed
  * iterations: all `iter`, or, with a tolerance, up to and including the
@@ -212,6 +214,9 @@ CILREG_API int cilreg_fgp_dev_ws(const float *d_input, float *d_output, float *i
 CILREG_API int cilreg_pd_dev_ws(const float *d_input, float *d_output, float *infovector, float lambdaPar, int iter,
                                 float epsil, float lipschitz_const, int methodTV, int nonneg, int dimX, int dimY,
                                 int dimZ, float *d_state, int warm, const cilreg_opts *opts, void *stream);
+CILREG_API int cilreg_sb_dev_ws(const float *d_input, float *d_output, float *infovector, float mu, int iter,
+                                float epsil, int methodTV, int dimX, int dimY, int dimZ, float *d_state, int warm,
+                                const cilreg_opts *opts, void *stream);
 CILREG_API int cilreg_tgv_dev_ws(const float *d_input, float *d_output, float *infovector, float lambda, float alpha1,
                                  float alpha0, int iter, float L2, float epsil, int dimX, int dimY, int dimZ,
                                  float *d_state, int warm, const cilreg_opts *opts, void *stream);

@@ -1,4 +1,4 @@
-"""Warm start (cilreg_{fgp,pd,tgv}_dev_ws): the iterate state in caller memory.
+"""Warm start (cilreg_{fgp,pd,sb,tgv}_dev_ws): the iterate state in caller memory.
 
 N iterations, then a warm call of M more from the returned state, must equal
 one call of N + M bit for bit -- output and state -- for every method, 2D and
@@ -26,6 +26,9 @@ def _run(method, x, iters, state, tstate, warm, **kw):
         return D.fgp_warm(x, 0.03, iters, state, tstate, warm=warm, nonneg=1, **kw)
     if method == "pd":
         return D.pd_warm(x, 0.03, iters, state, warm=warm, L=12.0, **kw)
+    if method == "sb":
+        kw.pop("precision", None)   # SB stores its fields in FP32
+        return D.sb_warm(x, 0.03, iters, state, warm=warm, **kw)
     return D.tgv_warm(x, 0.03, 1.0, 2.0, iters, state, warm=warm, **kw)
 
 
@@ -35,10 +38,13 @@ def _plain(method, x, iters, **kw):
         return D.fgp(x, 0.03, iters, nonneg=1, **kw)
     if method == "pd":
         return D.pd(x, 0.03, iters, L=12.0, **kw)
+    if method == "sb":
+        kw.pop("precision", None)
+        return D.sb(x, 0.03, iters, **kw)
     return D.tgv(x, 0.03, 1.0, 2.0, iters, **kw)
 
 
-@pytest.mark.parametrize("method", ["fgp", "pd", "tgv"])
+@pytest.mark.parametrize("method", ["fgp", "pd", "tgv", "sb"])
 @pytest.mark.parametrize("shape", SHAPES, ids=["3d", "2d"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_split_equals_single_call(method, shape, precision):
@@ -61,7 +67,7 @@ def test_split_equals_single_call(method, shape, precision):
         assert bool((y1 == ref).all()), tag + " vs plain"
 
 
-@pytest.mark.parametrize("method", ["fgp", "pd", "tgv"])
+@pytest.mark.parametrize("method", ["fgp", "pd", "tgv", "sb"])
 def test_tolerance_state_is_the_stop_iterations(method):
     # with a tolerance the state is that after the stop iteration: resuming
     # for M more equals a fixed-count solve of (stop + 1 + M) iterations
