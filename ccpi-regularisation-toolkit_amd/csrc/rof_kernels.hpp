@@ -214,6 +214,178 @@ __device__ inline void rof_body(const RofArgs<T>& a, const TileCoord& tc) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two iterations per launch on a 2D image (k -> k+2)
+// ---------------------------------------------------------------------------
+// A small image is launch-bound at one iteration per launch (512^2: ~1 us of
+// arithmetic in ~4.4 us per iteration).  One iteration's update at p reads u
+// on p's 3x3 neighbourhood (D1 at p and its mirrored -y neighbour, D2 at p
+// and its mirrored -x neighbour, each from u at its +-1 neighbours), so the
+// pass evaluates, for the 64 x TY tile at (x0, y0):
+//   A  u_k             on [x0-2, x0+65] x [y0-2, y0+TY+1]   (loaded)
+//   F  D1, D2 of k     on [x0-2, x0+64] x [y0-2, y0+TY]
+//   B  u_{k+1}         on [x0-1, x0+64] x [y0-1, y0+TY]
+//   G  D1, D2 of k+1   on [x0-1, x0+63] x [y0-1, y0+TY-1]
+//   T  u_{k+2}         on the tile (stored)
+// through LDS, every value by the expressions of rof_iter_kernel (same
+// fluxes, mirrors and operation order), so the results are bit-identical to
+// two one-iteration launches.  Region points outside the image take their
+// own slot for every neighbour (their values are never read by a point
+// inside it: mirrored neighbours of an image point lie in the image).
+template <typename T>
+struct Rof2Args {
+    const T* f;
+    const T* U;
+    T* Uout;
+    T lambda, tau;
+    int check0, check1;
+    double* partials0;
+    double* partials1;
+    const Ctrl* ctrl;
+    Geom g;
+};
+
+template <int TY>
+struct Rof2Shape {
+    static constexpr int AW = TX + 4, AH = TY + 4;   // A, origin (x0-2, y0-2)
+    static constexpr int FW = TX + 3, FH = TY + 3;   // F, origin (x0-2, y0-2)
+    static constexpr int BW = TX + 2, BH = TY + 2;   // B, origin (x0-1, y0-1)
+    static constexpr int GW = TX + 1, GH = TY + 1;   // G, origin (x0-1, y0-1)
+};
+
+// The slot (in a region of width w, height h, origin (ox, oy)) of the
+// mirrored +/-1 neighbours of image point (px, py); a point outside the image
+// or a neighbour outside the region gives the point's own slot.
+__device__ inline void rof2_nbr_slots(int px, int py, bool in, const Geom& g, int ox, int oy, int w, int h, int& sxp,
+                                      int& sxm, int& syp, int& sym) {
+    const int sx = px - ox, sy = py - oy;
+    sxp = sxm = sx;
+    syp = sym = sy;
+    if (in) {
+        sxp = mirror_p(px, g.nx) - ox;
+        sxm = mirror_m(px, g.nx) - ox;
+        syp = mirror_p(py, g.ny) - oy;
+        sym = mirror_m(py, g.ny) - oy;
+        if (sxp < 0 || sxp >= w) sxp = sx;
+        if (sxm < 0 || sxm >= w) sxm = sx;
+        if (syp < 0 || syp >= h) syp = sy;
+        if (sym < 0 || sym >= h) sym = sy;
+    }
+}
+
+template <int TY, typename T, typename I>
+__device__ inline void rof2_2d_body(const Rof2Args<T>& a, const TileCoord& tc) {
+    using Sh = Rof2Shape<TY>;
+    constexpr int NT = TX * TY;
+    const Geom& g = a.g;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const int x0 = tc.x0, y0 = tc.y0;
+    const I nx = (I)g.nx;
+    __shared__ T UA[Sh::AH][Sh::AW];
+    __shared__ T F1[Sh::FH][Sh::FW], F2[Sh::FH][Sh::FW];
+    __shared__ T UB[Sh::BH][Sh::BW];
+    __shared__ T G1[Sh::GH][Sh::GW], G2[Sh::GH][Sh::GW];
+    __shared__ double red[kRedSlots];
+
+    // A: u_k (points outside the image load a clamped stand-in)
+    for (int r = tid; r < Sh::AW * Sh::AH; r += NT) {
+        const int ry = r / Sh::AW, rx = r - ry * Sh::AW;
+        const int px = x0 - 2 + rx, py = y0 - 2 + ry;
+        const int cx = px < 0 ? 0 : (px >= g.nx ? g.nx - 1 : px), cy = py < 0 ? 0 : (py >= g.ny ? g.ny - 1 : py);
+        UA[ry][rx] = ldx<T>(a.U, (I)cy * nx + (I)cx);
+    }
+    __syncthreads();
+    // F: fluxes of iteration k
+    for (int r = tid; r < Sh::FW * Sh::FH; r += NT) {
+        const int ry = r / Sh::FW, rx = r - ry * Sh::FW;
+        const int px = x0 - 2 + rx, py = y0 - 2 + ry;   // A slot = (rx, ry)
+        const bool in = px >= 0 && py >= 0 && px < g.nx && py < g.ny;
+        int sxp, sxm, syp, sym;
+        rof2_nbr_slots(px, py, in, g, x0 - 2, y0 - 2, Sh::AW, Sh::AH, sxp, sxm, syp, sym);
+        T d1, d2, d3;
+        rof_flux_math<2, T>(UA[ry][rx], UA[ry][sxp], UA[ry][sxm], UA[syp][rx], UA[sym][rx], T(0), T(0), d1, d2, d3);
+        F1[ry][rx] = d1;
+        F2[ry][rx] = d2;
+    }
+    __syncthreads();
+    // B: u_{k+1}; the residual of iteration k on the tile
+    double sdd0 = 0.0, suu0 = 0.0, sdd1 = 0.0, suu1 = 0.0;
+    for (int r = tid; r < Sh::BW * Sh::BH; r += NT) {
+        const int ry = r / Sh::BW, rx = r - ry * Sh::BW;
+        const int px = x0 - 1 + rx, py = y0 - 1 + ry;   // F / A slot = (rx + 1, ry + 1)
+        const bool in = px >= 0 && py >= 0 && px < g.nx && py < g.ny;
+        // the mirrored -x / -y neighbours in F's slots
+        int fxm = rx + 1, fym = ry + 1;
+        if (in) {
+            fxm = mirror_m(px, g.nx) - (x0 - 2);
+            fym = mirror_m(py, g.ny) - (y0 - 2);
+        }
+        const T uc = UA[ry + 1][rx + 1];
+        const T dv = (F1[ry + 1][rx + 1] - F1[fym][rx + 1]) + (F2[ry + 1][rx + 1] - F2[ry + 1][fxm]);
+        const I i = (I)(in ? py : 0) * nx + (I)(in ? px : 0);
+        const T un = uc + a.tau * (a.lambda * dv - (uc - ldx<T>(a.f, i)));
+        UB[ry][rx] = un;
+        if (a.check0 && in && rx >= 1 && rx <= TX && ry >= 1 && ry <= TY) {
+            const T df = un - uc;
+            sdd0 += (double)df * (double)df;
+            suu0 += (double)un * (double)un;
+        }
+    }
+    __syncthreads();
+    // G: fluxes of iteration k+1
+    for (int r = tid; r < Sh::GW * Sh::GH; r += NT) {
+        const int ry = r / Sh::GW, rx = r - ry * Sh::GW;
+        const int px = x0 - 1 + rx, py = y0 - 1 + ry;   // B slot = (rx, ry)
+        const bool in = px >= 0 && py >= 0 && px < g.nx && py < g.ny;
+        int sxp, sxm, syp, sym;
+        rof2_nbr_slots(px, py, in, g, x0 - 1, y0 - 1, Sh::BW, Sh::BH, sxp, sxm, syp, sym);
+        T d1, d2, d3;
+        rof_flux_math<2, T>(UB[ry][rx], UB[ry][sxp], UB[ry][sxm], UB[syp][rx], UB[sym][rx], T(0), T(0), d1, d2, d3);
+        G1[ry][rx] = d1;
+        G2[ry][rx] = d2;
+    }
+    __syncthreads();
+    // T: u_{k+2}
+    {
+        const int lx = threadIdx.x, ly = threadIdx.y;
+        const int px = x0 + lx, py = y0 + ly;
+        if (px < g.nx && py < g.ny) {
+            const int gxm = mirror_m(px, g.nx) - (x0 - 1), gym = mirror_m(py, g.ny) - (y0 - 1);
+            const T uc = UB[ly + 1][lx + 1];
+            const T dv = (G1[ly + 1][lx + 1] - G1[gym][lx + 1]) + (G2[ly + 1][lx + 1] - G2[ly + 1][gxm]);
+            const I i = (I)py * nx + (I)px;
+            const T un = uc + a.tau * (a.lambda * dv - (uc - ldx<T>(a.f, i)));
+            stnt<T>(a.Uout, i, un);
+            if (a.check1) {
+                const T df = un - uc;
+                sdd1 += (double)df * (double)df;
+                suu1 += (double)un * (double)un;
+            }
+        }
+    }
+    if (a.check0) {
+        block_sum2(sdd0, suu0, red);
+        if (tid == 0) {
+            a.partials0[2 * tc.lin] = sdd0;
+            a.partials0[2 * tc.lin + 1] = suu0;
+        }
+        __syncthreads();
+    }
+    if (a.check1) {
+        block_sum2(sdd1, suu1, red);
+        if (tid == 0) {
+            a.partials1[2 * tc.lin] = sdd1;
+            a.partials1[2 * tc.lin + 1] = suu1;
+        }
+    }
+}
+
+template <int TY, typename T, typename I>
+__global__ __launch_bounds__(TX * TY) void rof2_2d_kernel(Rof2Args<T> a) {
+    if (solve_stopped(a.ctrl)) return;
+    rof2_2d_body<TY, T, I>(a, tile_of_block(a.g, TY));
+}
+
 // One iteration on one volume: the workgroup's tile comes from its block index.
 template <int DIM, int TY, typename T, typename I>
 __global__ __launch_bounds__(TX * TY) void rof_iter_kernel(RofArgs<T> a) {
