@@ -402,13 +402,15 @@ __device__ __attribute__((always_inline)) inline void fgp2_group_item(const FgpJ
     }
 }
 
+// Capped at 80 VGPRs with 32-bit offsets, like fgp2_iter_kernel (uncapped: 81
+// with FP32 P, 84 with 16-bit P, one workgroup per CU fewer).
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total, int level) {
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(fgp2_min_waves<T, I>()))) void fgp2_group_kernel(const FgpJob<T, S>* jobs, int njobs, int total, int level) {
     fgp2_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
 }
 
 template <int DIM, int TY, typename T, typename S, bool ISO, typename I>
-__global__ __launch_bounds__(TX * TY) void fgp2_persist_kernel(const FgpJob<T, S>* jobs, int njobs, PersistQ* q) {
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(fgp2_min_waves<T, I>()))) void fgp2_persist_kernel(const FgpJob<T, S>* jobs, int njobs, PersistQ* q) {
     persistent_run(jobs, njobs, q, [&](int level, int L) { fgp2_group_item<DIM, TY, T, S, ISO, I>(jobs, njobs, L, level); });
 }
 

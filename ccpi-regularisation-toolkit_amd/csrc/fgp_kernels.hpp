@@ -505,8 +505,17 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
     }
 }
 
+// Registers of the pass: with FP32 arithmetic and 32-bit offsets it is capped
+// at 80 VGPRs (six waves per SIMD: three 512-thread workgroups per CU); with
+// 16-bit P it would otherwise take 83 and lose one workgroup per CU (256^3
+// BF16: 0.114 -> 0.101 ms per iteration with the cap, 3 VGPRs spilled).  FP64
+// and 64-bit offsets are not capped: their bodies need far more.
+template <typename T, typename I>
+constexpr int fgp2_min_waves() { return (std::is_same<T, float>::value && sizeof(I) == 4) ? 6 : 1; }
+
 template <int TY, typename T, typename S, bool ISO, typename I, int DIM = 3>
-__global__ __launch_bounds__(TX * TY) void fgp2_iter_kernel(Fgp2Args<T, S> a) {
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(fgp2_min_waves<T, I>())))
+void fgp2_iter_kernel(Fgp2Args<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
     fgp2_body<TY, T, S, ISO, I, DIM>(a, tile_of_block(a.g, TY));
 }

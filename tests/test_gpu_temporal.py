@@ -298,3 +298,38 @@ def test_depth2_stop_on_either_iteration_of_a_pass(solver):
     # must see both positions
     need = {0} if solver == "tgv" else {0, 1}
     assert need <= parities, f"stop positions exercised: {parities}"
+
+
+# ROF 2D images below 2^20 pixels run two iterations per launch by default
+# (depth=2 forces it for any 2D image): iteration k's fluxes and u_{k+1} on the
+# tile and a one-point ring, then iteration k+1 on the tile, from LDS.  A stop
+# on the pass's first iteration is recomputed by one one-iteration launch.
+ROF2D_SHAPES = [(1, 1), (2, 1), (1, 70), (70, 1), (3, 5), (8, 64), (9, 65), (7, 63), (17, 127), (130, 5),
+                (512, 512), (700, 1000)]
+
+
+@pytest.mark.parametrize("shape", ROF2D_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_rof2d_depth2_matches_depth1(shape):
+    import torch
+    from ccpi.filters import device as D
+    x = torch.from_numpy(np.random.default_rng(43).random(shape, dtype=np.float32)).cuda()
+    for iters in (1, 2, 3, 8, 41):
+        for lam in (0.02, 0.2):
+            kw = dict(lam=lam, iters=iters, tau=0.001)
+            _same(D.rof(x, depth=1, **kw), D.rof(x, depth=2, **kw), f"rof {shape} iters={iters} lam={lam}")
+            _same(D.rof(x, depth=1, **kw), D.rof(x, **kw), f"rof default {shape} iters={iters} lam={lam}")
+
+
+def test_rof2d_depth2_stop_on_either_iteration_of_a_pass():
+    import torch
+    from ccpi.filters import device as D
+    x = torch.from_numpy(np.random.default_rng(44).random((96, 200), dtype=np.float32)).cuda()
+    parities = set()
+    for ce in (1, 3, 5):
+        for tol in np.geomspace(3e-2, 1e-5, 24):
+            kw = dict(lam=0.02, iters=300, tau=0.001, tol=float(tol), check_every=ce)
+            a = D.rof(x, depth=1, **kw)
+            _same(a, D.rof(x, depth=2, **kw), f"rof tol={tol} check_every={ce}")
+            if int(a[1][0]) < 300:
+                parities.add(int(a[1][0]) % 2)
+    assert parities == {0, 1}, f"stop positions exercised: {parities}"
