@@ -229,35 +229,63 @@ __device__ inline T sb_combine(const SbArgs<T>& a, T db, T usum, T fv) {
     return a.nc * acc;
 }
 
-template <int DIM, int TY, typename T, bool ANISO, typename I>
-__global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
+// Ring point h (0 <= h < 3 (TX+3) + 3 TY) of the region, in region coordinates.
+template <int TY>
+__device__ inline void sb_ring_point(int h, int& rx, int& ry) {
+    constexpr int BW = TX + 3;
+    if (h < 3 * BW) {
+        const int row = h / BW;
+        rx = h - row * BW;
+        ry = row == 0 ? 0 : TY + row;          // rows y0-1, y0+TY, y0+TY+1
+    } else {
+        const int k = h - 3 * BW, col = k / TY;
+        rx = col == 0 ? 0 : TX + col;          // columns x0-1, x0+TX, x0+TX+1
+        ry = k - col * TY + 1;
+    }
+}
+
+// RW = 0: TX x TY threads, each the own tile point (slot 0) and NR of them
+// also a ring point (slot 1, on the low or high waves by block index).
+// RW > 0: TX x TY + 64 RW threads, the first TX x TY the tile, the RW waves
+// after them the ring, one point per thread.
+template <int TY, int RW>
+constexpr int sb_fused_threads() { return TX * TY + 64 * RW; }
+
+template <int DIM, int TY, typename T, bool ANISO, typename I, int RW = 0>
+__global__ __launch_bounds__((sb_fused_threads<TY, RW>())) void sb_fused_kernel(SbArgs<T> a) {
     if (solve_stopped(a.ctrl)) return;
     const Geom& g = a.g;
     const TileCoord tc = tile_of_block(g, TY);
     constexpr int BW = TX + 3, BH = TY + 3, NR = 3 * BW + 3 * TY, NT = TX * TY;
+    constexpr int NS = RW ? 1 : 2;   // slots per thread
+    static_assert(RW == 0 || 64 * RW >= NR, "ring waves must hold the ring");
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
 
-    int bx[2], by[2];
-    bx[0] = lx + 1;
-    by[0] = ly + 1;
-    const int hbase = (tc.lin & 1) ? NT - NR : 0;
-    const int h = tid - hbase;
-    const bool ring = h >= 0 && h < NR;
-    {
+    int bx[2] = {0, 0}, by[2] = {0, 0};
+    int nslot;
+    bool tile0;   // slot 0 is a tile point
+    if (RW) {
+        tile0 = tid < NT;
+        const int h = tid - NT;
+        const bool ring = !tile0 && h < NR;
         int rx = 0, ry = 0;
-        if (h < 3 * BW) {
-            const int row = h / BW;
-            rx = h - row * BW;
-            ry = row == 0 ? 0 : TY + row;          // rows y0-1, y0+TY, y0+TY+1
-        } else {
-            const int k = h - 3 * BW, col = k / TY;
-            rx = col == 0 ? 0 : TX + col;          // columns x0-1, x0+TX, x0+TX+1
-            ry = k - col * TY + 1;
-        }
-        bx[1] = ring ? rx : 0;
-        by[1] = ring ? ry : 0;
+        if (ring) sb_ring_point<TY>(h, rx, ry);
+        bx[0] = tile0 ? lx + 1 : rx;
+        by[0] = tile0 ? ly + 1 : ry;
+        nslot = (tile0 || ring) ? 1 : 0;
+    } else {
+        tile0 = true;
+        bx[0] = lx + 1;
+        by[0] = ly + 1;
+        const int hbase = (tc.lin & 1) ? NT - NR : 0;
+        const int h = tid - hbase;
+        const bool ring = h >= 0 && h < NR;
+        int rx = 0, ry = 0;
+        if (ring) sb_ring_point<TY>(h, rx, ry);
+        bx[1] = rx;
+        by[1] = ry;
+        nslot = ring ? 2 : 1;
     }
-    const int nslot = ring ? 2 : 1;
     int X[2], Y[2], cx[2], cy[2];
     bool in[2];
 #pragma unroll
@@ -268,10 +296,17 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         cx[s] = in[s] ? X[s] : 0;   // clamped stand-in
         cy[s] = in[s] ? Y[s] : 0;
     }
-    const bool own = in[0];
-    // slot 1 also evaluates U2 on the +x column and the +y row
-    const bool u2halo = ring && in[1] &&
-                        ((by[1] == TY + 1 && bx[1] >= 1 && bx[1] <= TX) || (bx[1] == TX + 1 && by[1] >= 1 && by[1] <= TY));
+    const bool own = tile0 && in[0];
+    // the ring slot also evaluates U2 on the +x column and the +y row
+    const int rs = RW ? 0 : 1;
+    const bool u2halo = nslot > rs && !(RW && tile0) && in[rs] &&
+                        ((by[rs] == TY + 1 && bx[rs] >= 1 && bx[rs] <= TX) ||
+                         (bx[rs] == TX + 1 && by[rs] >= 1 && by[rs] <= TY));
+    // slots that evaluate U2 (points inside the volume: their mirrored
+    // neighbours are region slots)
+    bool u2ok[2];
+    u2ok[0] = RW ? (tile0 ? own : u2halo) : own;
+    u2ok[1] = RW ? false : u2halo;
     // mirrored x / y neighbour slots (SB's rule: +1, or -1 at the last index)
     int sxp[2], sxm[2], syp[2], sym[2];
 #pragma unroll
@@ -294,7 +329,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
     T u0m[2] = {}, u0c[2] = {}, d2m[2] = {}, b2m[2] = {};
     const int zstart = z0 >= 2 ? z0 - 2 : 0;
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
         if (s < nslot) {
             const I col = (I)cy[s] * nx + (I)cx[s];
             u0c[s] = ldx<T>(a.Uin, (I)zstart * sxy + col);
@@ -315,7 +350,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         const bool s1 = zi < nz;
         if (s1) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 if (s < nslot) {
                     const I col = (I)cy[s] * nx + (I)cx[s];
                     const I c = (I)zi * sxy + col;
@@ -339,7 +374,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         // or from memory where the mirrored neighbour lies outside the region
         if (s1) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 if (s < nslot) {
                     const I zo = (I)zi * sxy, row = (I)cy[s] * nx;
                     const bool xm_in = sxm[s] >= 0 && sxm[s] < BW, xp_in = sxp[s] >= 0 && sxp[s] < BW;
@@ -375,9 +410,8 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
         const int zc = zi - 1;
         if (zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 && zc < nz) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                // only points inside the volume: their mirrored neighbours are region slots
-                if (s == 0 ? own : u2halo) {
+            for (int s = 0; s < NS; ++s) {
+                if (u2ok[s]) {
                     const int b = zc & 3;
                     T us = U1p[b][by[s]][sxp[s]] + U1p[b][by[s]][sxm[s]] + U1p[b][syp[s]][bx[s]] +
                            U1p[b][sym[s]][bx[s]];
@@ -437,7 +471,7 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
             bq1[d] = s1 ? bv[0][d] : (T)0;
         }
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < NS; ++s) {
             if (s1) {
                 dbz[s] = dbn[s];
                 fz[s] = fn[s];
@@ -459,13 +493,18 @@ __global__ __launch_bounds__(TX * TY) void sb_fused_kernel(SbArgs<T> a) {
     }
 }
 
+// The 3D pass on 64 x 8 tiles runs its ring on four waves of its own.
+template <int DIM, int TY>
+constexpr int sb_ring_waves() { return (DIM == 3 && TY == 8) ? 4 : 0; }
+
 template <int DIM, int TY, typename T>
 static const void* sb_fused_ptr(bool aniso, bool small) {
+    constexpr int RW = sb_ring_waves<DIM, TY>();
     if (small)
-        return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint32_t>
-                     : (const void*)sb_fused_kernel<DIM, TY, T, false, uint32_t>;
-    return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint64_t>
-                 : (const void*)sb_fused_kernel<DIM, TY, T, false, uint64_t>;
+        return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint32_t, RW>
+                     : (const void*)sb_fused_kernel<DIM, TY, T, false, uint32_t, RW>;
+    return aniso ? (const void*)sb_fused_kernel<DIM, TY, T, true, uint64_t, RW>
+                 : (const void*)sb_fused_kernel<DIM, TY, T, false, uint64_t, RW>;
 }
 
 template <typename T, typename S>
@@ -491,7 +530,8 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
     Geom g;
     if (fused)
         g = dim == 2 ? make_geom(nx, ny, nz, 4, 1)
-                     : (fty == 8 ? plan_geom(sb_fused_ptr<3, 8, T>(aniso, small), TX * 8, nx, ny, nz, 8, opts.zchunk, 2.0)
+                     : (fty == 8 ? plan_geom(sb_fused_ptr<3, 8, T>(aniso, small), sb_fused_threads<8, sb_ring_waves<3, 8>()>(),
+                                             nx, ny, nz, 8, opts.zchunk, 2.0)
                                  : plan_geom(sb_fused_ptr<3, 4, T>(aniso, small), TX * 4, nx, ny, nz, 4, opts.zchunk, 2.0));
     else
         // the two-kernel path is built for 64 x 4 tiles
@@ -556,7 +596,8 @@ int sb_solve(const T* d_f, T* d_out, T mu_in, int iter, T epsil, int method, int
             a.Uin = (const T*)ar.at((k & 1) ? oU1 : oU0);
             a.Uout = (T*)ar.at((k & 1) ? oU0 : oU1);
             a.check = check;
-            dim3 grid(g.nblocks), block(TX, fty);
+            const int rw = (dim == 3 && fty == 8) ? sb_ring_waves<3, 8>() : 0;
+            dim3 grid(g.nblocks), block(TX, fty + rw);
             SbArgs<T> arg = a;
             void* args[] = {&arg};
             const void* kp = dim == 3 ? (fty == 8 ? sb_fused_ptr<3, 8, T>(aniso, small) : sb_fused_ptr<3, 4, T>(aniso, small))
