@@ -73,7 +73,9 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
         *out = TuneChoice{4, 1, two ? 2 : 1, 0.0};
         return 0;
     }
-    const Key key{method, base.precision, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX, (ny + 31) / 32 * 32,
+    // ROF has no dual field: one choice serves every storage precision
+    const int kprec = method == CILREG_ROF ? CILREG_FP32 : base.precision;
+    const Key key{method, kprec, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX, (ny + 31) / 32 * 32,
                   (nz + 31) / 32 * 32};
     {
         std::lock_guard<std::mutex> lk(g_mu);
