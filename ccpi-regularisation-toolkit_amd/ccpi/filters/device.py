@@ -12,6 +12,7 @@ Options beyond the reference (see include/cilreg_ext.h):
 ``zchunk`` (tiling; 0 = automatic).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -84,6 +85,45 @@ def autotune(method, shape, method_tv=0, **kw):
                            ctypes.byref(ms)) != 0:
         raise RuntimeError(f"cilreg_autotune failed: {last_error()}")
     return {"tile_y": best.tile_y, "zchunk": best.zchunk, "depth": best.temporal_depth, "ms_per_iter": ms.value}
+
+
+def _tune_lib():
+    lib = require_gpu_lib()
+    for name in ("cilreg_tune_save", "cilreg_tune_load"):
+        getattr(lib, name).argtypes = [ctypes.c_char_p]
+        getattr(lib, name).restype = ctypes.c_int
+    lib.cilreg_tune_entries.argtypes = []
+    lib.cilreg_tune_entries.restype = ctypes.c_int
+    lib.cilreg_tune_clear.argtypes = []
+    lib.cilreg_tune_clear.restype = None
+    return lib
+
+
+def tune_save(path):
+    """Write the autotuner's cached choices to `path`; the number written."""
+    n = _tune_lib().cilreg_tune_save(os.fsencode(path))
+    if n < 0:
+        raise RuntimeError(f"cilreg_tune_save failed: {last_error()}")
+    return n
+
+
+def tune_load(path):
+    """Merge an autotuner cache file into this process's choices; the number
+    read.  A file with a malformed entry is rejected whole (RuntimeError)."""
+    n = _tune_lib().cilreg_tune_load(os.fsencode(path))
+    if n < 0:
+        raise RuntimeError(f"cilreg_tune_load failed: {last_error()}")
+    return n
+
+
+def tune_entries():
+    """Number of autotuner choices cached in this process."""
+    return _tune_lib().cilreg_tune_entries()
+
+
+def tune_clear():
+    """Forget this process's autotuner choices (cache files are untouched)."""
+    _tune_lib().cilreg_tune_clear()
 
 
 _vp = ctypes.c_void_p

@@ -12,10 +12,20 @@
 // for a size class once.  Every candidate computes the same values bit for bit (the results
 // do not depend on the tiling or the pass width), so tuning never changes an
 // answer -- only, with a tolerance, the order of the FP64 residual sums.
+//
+// The choices persist in a text file when the caller asks for it
+// (cilreg_tune_save / cilreg_tune_load, or CILREG_TUNE_CACHE=<path>: loaded
+// before the first tuning of the process and rewritten after every new
+// choice).  Each entry names the device it was measured on (architecture and
+// CU count) and is used only on such a device.
 #include "solvers.hpp"
 #include "tune.hpp"
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -24,14 +34,123 @@ namespace cilreg {
 namespace {
 
 struct Key {
+    std::string device;   // architecture name and CU count, e.g. "gfx950:256"
     int method, precision, iso, fixed_ty, nx, ny, nz;
     bool operator<(const Key& o) const {
-        return std::tie(method, precision, iso, fixed_ty, nx, ny, nz) <
-               std::tie(o.method, o.precision, o.iso, o.fixed_ty, o.nx, o.ny, o.nz);
+        return std::tie(device, method, precision, iso, fixed_ty, nx, ny, nz) <
+               std::tie(o.device, o.method, o.precision, o.iso, o.fixed_ty, o.nx, o.ny, o.nz);
     }
 };
 std::mutex g_mu;
 std::map<Key, TuneChoice> g_cache;
+bool g_env_loaded = false;   // CILREG_TUNE_CACHE read (guarded by g_mu)
+
+constexpr const char* kCacheHeader = "# cilreg autotune cache v1";
+
+// Identity of the current device for the cache: its architecture (the first
+// field of gcnArchName) and its CU count.
+int device_identity(std::string* id) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        set_error("autotune: no device properties");
+        return -1;
+    }
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    if (arch.empty()) arch = "unknown";
+    *id = arch + ":" + std::to_string(prop.multiProcessorCount);
+    return 0;
+}
+
+// Writes every cached choice to path (through path.tmp and a rename, so a
+// reader never sees half a file); the number written, or -1.  Caller holds g_mu.
+int save_locked(const char* path) {
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) {
+        set_error("autotune cache: cannot write %s", tmp.c_str());
+        return -1;
+    }
+    std::fprintf(f, "%s\n# device method precision iso grouped nx ny nz tile_y zchunk depth ms_per_iter\n", kCacheHeader);
+    int n = 0;
+    for (const auto& kv : g_cache) {
+        const Key& k = kv.first;
+        const TuneChoice& c = kv.second;
+        std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %.17g\n", k.device.c_str(), k.method, k.precision, k.iso,
+                     k.fixed_ty, k.nx, k.ny, k.nz, c.tile_y, c.zchunk, c.depth, c.ms_per_iter);
+        ++n;
+    }
+    const bool ok = std::fflush(f) == 0 && !std::ferror(f);
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        set_error("autotune cache: cannot write %s", path);
+        return -1;
+    }
+    return n;
+}
+
+// Reads path into the cache; entries already cached are replaced.  The whole
+// file is checked before anything is merged: a malformed line rejects it.
+// missing_ok: a file that does not exist loads nothing.  Caller holds g_mu.
+int load_locked(const char* path, bool missing_ok) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) {
+        if (missing_ok) return 0;
+        set_error("autotune cache: cannot read %s", path);
+        return -1;
+    }
+    std::vector<std::pair<Key, TuneChoice>> got;
+    char line[512];
+    int lineno = 0;
+    bool header = false, bad = false;
+    while (std::fgets(line, sizeof line, f)) {
+        ++lineno;
+        if (!std::strchr(line, '\n') && !std::feof(f)) { bad = true; break; }   // line too long
+        if (lineno == 1) {
+            header = std::strncmp(line, kCacheHeader, std::strlen(kCacheHeader)) == 0;
+            if (!header) break;
+            continue;
+        }
+        if (line[0] == '#' || line[0] == '\n') continue;
+        char dev[128];
+        Key k;
+        TuneChoice c;
+        int used = 0;
+        const int m = std::sscanf(line, "%127s %d %d %d %d %d %d %d %d %d %d %lf %n", dev, &k.method, &k.precision,
+                                  &k.iso, &k.fixed_ty, &k.nx, &k.ny, &k.nz, &c.tile_y, &c.zchunk, &c.depth,
+                                  &c.ms_per_iter, &used);
+        const bool valid = m == 12 && line[used] == '\0' && k.method >= CILREG_ROF && k.method <= CILREG_TGV &&
+                           k.precision >= CILREG_FP32 && k.precision <= CILREG_FP16 && (k.iso == 0 || k.iso == 1) &&
+                           (k.fixed_ty == 0 || k.fixed_ty == 1) && k.nx > 0 && k.ny > 0 && k.nz > 0 &&
+                           (c.tile_y == 4 || c.tile_y == 8) && c.zchunk >= 0 && (c.depth == 1 || c.depth == 2) &&
+                           c.ms_per_iter >= 0;
+        if (!valid) { bad = true; break; }
+        k.device = dev;
+        got.emplace_back(k, c);
+    }
+    std::fclose(f);
+    if (!header) {
+        set_error("autotune cache %s: not a cilreg autotune cache (first line must be '%s')", path, kCacheHeader);
+        return -1;
+    }
+    if (bad) {
+        set_error("autotune cache %s: malformed entry on line %d", path, lineno);
+        return -1;
+    }
+    for (const auto& e : got) g_cache[e.first] = e.second;
+    return (int)got.size();
+}
+
+// Loads CILREG_TUNE_CACHE once per process (a missing file is an empty
+// cache).  Caller holds g_mu.
+void env_load_locked() {
+    if (g_env_loaded) return;
+    g_env_loaded = true;
+    const char* path = std::getenv("CILREG_TUNE_CACHE");
+    if (path && *path) (void)load_locked(path, true);   // an unreadable cache is reported and ignored
+}
 
 // deterministic pseudo-random volume in [0, 1) (hash of the index)
 __global__ __launch_bounds__(256) void probe_fill_kernel(float* p, size_t n) {
@@ -75,10 +194,13 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
     }
     // ROF has no dual field: one choice serves every storage precision
     const int kprec = method == CILREG_ROF ? CILREG_FP32 : base.precision;
-    const Key key{method, kprec, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX, (ny + 31) / 32 * 32,
-                  (nz + 31) / 32 * 32};
+    std::string dev_id;
+    if (device_identity(&dev_id) != 0) return -1;
+    const Key key{dev_id, method, kprec, iso ? 1 : 0, fixed_ty ? 1 : 0, (nx + TX - 1) / TX * TX,
+                  (ny + 31) / 32 * 32, (nz + 31) / 32 * 32};
     {
         std::lock_guard<std::mutex> lk(g_mu);
+        env_load_locked();
         auto it = g_cache.find(key);
         if (it != g_cache.end()) {
             *out = it->second;
@@ -148,6 +270,8 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[key] = best;
     *out = best;
+    const char* path = std::getenv("CILREG_TUNE_CACHE");
+    if (path && *path) (void)save_locked(path);   // a failed record is reported; the choice stands
     return 0;
 }
 
@@ -183,4 +307,26 @@ extern "C" CILREG_API int cilreg_autotune(int method, int dimX, int dimY, int di
     }
     if (ms_per_iter) *ms_per_iter = c.ms_per_iter;
     return 0;
+}
+
+extern "C" CILREG_API int cilreg_tune_save(const char* path) {
+    if (!path || !*path) { set_error("cilreg_tune_save: no path"); return -1; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    return save_locked(path);
+}
+
+extern "C" CILREG_API int cilreg_tune_load(const char* path) {
+    if (!path || !*path) { set_error("cilreg_tune_load: no path"); return -1; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    return load_locked(path, false);
+}
+
+extern "C" CILREG_API int cilreg_tune_entries(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return (int)g_cache.size();
+}
+
+extern "C" CILREG_API void cilreg_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache.clear();
 }

@@ -232,6 +232,24 @@ CILREG_API int cilreg_tgv_dev_ws(const float *d_input, float *d_output, float *i
 CILREG_API int cilreg_autotune(int method, int dimX, int dimY, int dimZ, int methodTV, const cilreg_opts *base,
                                cilreg_opts *best, double *ms_per_iter);
 
+/* The autotuner's choices as a text file (no counterpart in the reference,
+ * whose tile sizes are compile-time constants).  First line
+ * "# cilreg autotune cache v1", then one entry per line:
+ *   device method precision iso grouped nx ny nz tile_y zchunk depth ms_per_iter
+ * device = architecture and CU count ("gfx950:256"); an entry is used only on
+ * such a device.  cilreg_tune_save writes every cached choice (through
+ * path.tmp and a rename) and returns the number written; cilreg_tune_load
+ * merges a file into the process's cache (replacing entries with the same
+ * key) and returns the number read -- a file with a malformed line is
+ * rejected whole (-1, cilreg_last_error names the line).  With the
+ * environment variable CILREG_TUNE_CACHE=<path> the file is loaded before the
+ * process's first tuning (a missing file is an empty cache) and rewritten
+ * after every new choice.  These calls touch no device. */
+CILREG_API int cilreg_tune_save(const char *path);
+CILREG_API int cilreg_tune_load(const char *path);
+CILREG_API int cilreg_tune_entries(void);  /* choices cached in this process */
+CILREG_API void cilreg_tune_clear(void);   /* forget them (files are untouched) */
+
 /* Runs every job of the batch to completion with grouped kernels: one launch
  * per iteration level and method group covers all volumes of that group, so
  * many small or uneven volumes fill the device together.  Unless

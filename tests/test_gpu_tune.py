@@ -69,3 +69,46 @@ def test_tuned_batch_equals_single_solves():
             ref = D.tgv(x, 0.03, 1.0, 2.0, 17)
         assert np.array_equal(info, ref[1]), m
         assert bool((y == ref[0]).all()), m
+
+
+def test_tuning_cache_file_round_trip(tmp_path):
+    """Choices saved to a file and loaded into a cleared cache are used
+    without timing anything; the entry names this device."""
+    from ccpi.filters import device as D
+    shape = (28, 44, 80)
+    first = D.autotune("fgp", shape, precision="bf16")
+    path = tmp_path / "tune.txt"
+    assert D.tune_save(path) >= 1
+    rows = [l.split() for l in open(path) if not l.startswith("#")]
+    assert all(r[0].startswith("gfx") and ":" in r[0] for r in rows)
+    D.tune_clear()
+    assert D.tune_entries() == 0
+    assert D.tune_load(path) == len(rows)
+    t0 = time.perf_counter()
+    again = D.autotune("fgp", shape, precision="bf16")
+    assert time.perf_counter() - t0 < 0.05
+    assert again == first
+
+
+def test_tuning_cache_environment_variable(tmp_path):
+    """CILREG_TUNE_CACHE: a fresh process records its choices there, and a
+    second process reuses them without timing."""
+    import json
+    import os
+    import subprocess
+    import sys
+    path = tmp_path / "env_tune.txt"
+    code = ("import json, sys, time; sys.path.insert(0, %r); from ccpi.filters import device as D; "
+            "t = time.perf_counter(); c = D.autotune('pd', (30, 40, 70)); "
+            "print(json.dumps([c, time.perf_counter() - t]))")
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "ccpi-regularisation-toolkit_amd")
+    env = dict(os.environ, CILREG_TUNE_CACHE=str(path))
+    runs = []
+    for _ in range(2):
+        r = subprocess.run([sys.executable, "-c", code % pkg], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    assert path.exists() and len([l for l in open(path) if not l.startswith("#")]) >= 1
+    assert runs[1][0] == runs[0][0]
+    assert runs[1][1] < 0.05
