@@ -832,7 +832,11 @@ public:
         // two-iteration passes: FGP, PD and TGV in 3D, FGP on small 2D images
         const bool two = (j->pub.method != CILREG_ROF && j->dim == 3) ||
                          (j->pub.method == CILREG_FGP && j->n < kFgp2dPassMaxPixels);
-        int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : 16) : 1;
+        // grouped 3D jobs march 32-plane chunks: measured on the 64-volume
+        // batch against 16-plane chunks, per-job tuned depths and a depth
+        // planned for the group's workgroup rounds, 32 planes gave the
+        // shortest batch (deeper chunks leave longer tails at every level)
+        int zc = j->dim == 3 ? (opts_.zchunk > 0 ? opts_.zchunk : kGroupZChunk) : 1;
         j->width = two && opts_.depth != 1 ? 2 : 1;
         if (opts_.autotune && j->dim == 3 && opts_.zchunk == 0 && opts_.depth == 0) {
             TuneChoice c;

@@ -212,6 +212,9 @@ struct Opts {
 // traffic it saves (2048^2: 30.5 vs 32.3 us).
 constexpr size_t kFgp2dPassMaxPixels = (size_t)1 << 20;
 
+// z planes per chunk of a grouped (batched / served) 3D job.
+constexpr int kGroupZChunk = 32;
+
 // Per-iteration kernel timing of the last profiled solve on this thread.
 struct ProfileStats { double mean_ms = 0, min_ms = 0; int count = 0, passes = 0; };
 ProfileStats& thread_profile();

@@ -4,7 +4,7 @@
 // For a (method, dual storage, TV type, volume extent) it runs a short
 // fixed-count solve (four iterations) of a synthetic volume of that extent
 // for each candidate -- tile height 8 or 4, z chunk planned by occupancy or
-// 8 / 16 / 32 planes (grouped jobs: 16 / 32 planes after an untimed
+// 8 / 16 / 32 planes (grouped jobs: the engine's 32-plane chunks after an untimed
 // warm-up solve, eight timed iterations), one or two iterations per pass
 // where the method has both -- takes the per-iteration kernel time from HIP
 // events (the solves' own profiling), and keeps the fastest.  Results are cached per process,
@@ -235,10 +235,11 @@ int autotune(int method, int nx, int ny, int nz, int iso, const Opts& base, bool
     std::vector<int> tys = fixed_ty ? std::vector<int>{8} : std::vector<int>{8, 4};
     std::vector<int> zcs;
     if (!fixed_ty) zcs.push_back(0);   // the occupancy-planned depth
-    // grouped: deep chunks only -- shallow ones win alone by filling the
-    // device with more workgroups, which a full group does not need
-    for (int z : fixed_ty ? std::initializer_list<int>{16, 32} : std::initializer_list<int>{8, 16, 32})
-        if (z <= nz) zcs.push_back(z);
+    // grouped: the engine's fixed chunk depth (only the pass width is tuned)
+    if (fixed_ty) zcs.push_back(kGroupZChunk < nz ? kGroupZChunk : nz);
+    else
+        for (int z : {8, 16, 32})
+            if (z <= nz) zcs.push_back(z);
     if (zcs.empty()) zcs.push_back(nz);
     TuneChoice best{8, fixed_ty ? zcs.front() : 0, depths.front(), 1e300};
     const int iters = fixed_ty ? 8 : 4;
