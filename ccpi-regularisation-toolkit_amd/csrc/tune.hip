@@ -48,11 +48,23 @@ bool g_env_loaded = false;   // CILREG_TUNE_CACHE read (guarded by g_mu)
 constexpr const char* kCacheHeader = "# cilreg autotune cache v1";
 
 // Identity of the current device for the cache: its architecture (the first
-// field of gcnArchName) and its CU count.
+// field of gcnArchName) and its CU count; looked up once per device.
 int device_identity(std::string* id) {
+    static std::mutex mu;
+    static std::map<int, std::string> known;
     int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("autotune: no current device");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) {
+        *id = it->second;
+        return 0;
+    }
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
         set_error("autotune: no device properties");
         return -1;
     }
@@ -60,7 +72,9 @@ int device_identity(std::string* id) {
     const size_t colon = arch.find(':');
     if (colon != std::string::npos) arch.resize(colon);
     if (arch.empty()) arch = "unknown";
-    *id = arch + ":" + std::to_string(prop.multiProcessorCount);
+    for (char& ch : arch)
+        if (ch == ' ') ch = '_';   // one whitespace-free field of the file
+    *id = known[dev] = arch + ":" + std::to_string(prop.multiProcessorCount);
     return 0;
 }
 
