@@ -246,14 +246,23 @@ __device__ inline bool solve_stopped(const Ctrl* c) { return c != nullptr && c->
 // so on that interval it returns sqrtf's bits (checked over every float of
 // it by cilreg_check_sqrt_rn).  A wave with any lane outside the interval
 // (zero, denormal, negative, infinite, NaN) calls sqrtf.
+//
+// The two decisions are read from the sign bits of (r -/+ 1 ulp) r - x, the
+// negated residuals (an exact zero is +0, so "x - (r - 1 ulp) r <= 0" is a
+// clear sign bit): the root is the bit pattern of r - 1 ulp plus one for
+// each neighbour whose square lies below x (at most r + 1 ulp: the second
+// implies the first).  Integer shifts and one add instead of compares and
+// selects, which on this target wait two cycles each between the compare and
+// its use.
 __device__ inline float sqrt_rn(float x) {
     const bool fast = x >= 0x1p-96f && x <= 0x1.fffffep127f;   // false for NaN
     if (__builtin_amdgcn_ballot_w64(!fast)) return sqrtf(x);
     const float r = __builtin_amdgcn_sqrtf(x);
-    const float rm = __uint_as_float(__float_as_uint(r) - 1u), rp = __uint_as_float(__float_as_uint(r) + 1u);
-    float y = __builtin_fmaf(-rm, r, x) <= 0.f ? rm : r;
-    y = __builtin_fmaf(-rp, r, x) > 0.f ? rp : y;
-    return y;
+    const uint32_t ri = __float_as_uint(r);
+    const float rm = __uint_as_float(ri - 1u), rp = __uint_as_float(ri + 1u);
+    const uint32_t above_m = __float_as_uint(__builtin_fmaf(rm, r, -x)) >> 31;   // 1: x - (r - 1 ulp) r > 0
+    const uint32_t above_p = __float_as_uint(__builtin_fmaf(rp, r, -x)) >> 31;   // 1: x - (r + 1 ulp) r > 0
+    return __uint_as_float(ri - 1u + above_m + above_p);
 }
 
 // ---------------------------------------------------------------------------
