@@ -7,7 +7,10 @@
 * BF16 / FP16 storage of the dual and auxiliary fields (FGP: P, P_prev; PD: P;
   TGV: P, Q, v_bar) with FP32 arithmetic: relative L2 error of the output
   against the FP64 solve, with the bounds stated below (FP32 storage 1e-5,
-  FP16 1e-3, BF16 3e-3 at these sizes and iteration counts).
+  FP16 1e-3, BF16 3e-3 at these sizes and iteration counts).  The BF16 bound
+  is looser than the 1e-3 the precision target asks for: BF16 meets 1e-3 for
+  TGV and PD but not for FGP, whose dual P lies on the unit ball where 8
+  significand bits give about 2^-9 per element; FP16 meets it everywhere.
 """
 import numpy as np
 import pytest
