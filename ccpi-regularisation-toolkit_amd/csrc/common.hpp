@@ -265,6 +265,22 @@ __device__ inline float sqrt_rn(float x) {
     return __uint_as_float(ri - 1u + above_m + above_p);
 }
 
+// Correctly rounded double square root for x in [2^-1000, 2^1000] (no
+// scaling needed there): an estimate g = x rsq(x), h = rsq(x) / 2, one
+// coupled Newton step on both (e = 1/2 - g h), then two residual
+// corrections g += (x - g^2) h, every step a fused multiply-add.
+__device__ inline double sqrt_dn(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 // ---------------------------------------------------------------------------
 // exact restatements of the reference's mixed-precision expressions
 // ---------------------------------------------------------------------------
@@ -273,8 +289,17 @@ template <> struct Mixed<float> {
     // sqrtf((float)((double)s + eps)): ROF_TV_core.c:160,199,223,262 (eps 1e-8;
     // LLT_ROF_core.c:256,296,320,359 with 1e-12)
     __device__ static inline float rof_sqrt_f(float s, double eps = 1.0e-8) { return sqrt_rn((float)((double)s + eps)); }
-    // (float)sqrt((double)s + eps): the 3D D1, ROF_TV_core.c:136, LLT_ROF_core.c:232
-    __device__ static inline float rof_sqrt_d(float s, double eps = 1.0e-8) { return (float)sqrt((double)s + eps); }
+    // (float)sqrt((double)s + eps): the 3D D1, ROF_TV_core.c:136, LLT_ROF_core.c:232.
+    // s >= 0 (squares and squared minmods), so x = s + eps lies in
+    // [eps, 2^129) unless s is infinite or NaN: in range, sqrt_dn gives sqrt's
+    // bits without its scaling and special-value patches (checked for every
+    // float s with both eps values by cilreg_check_rof_sqrt_d); a wave with an
+    // out-of-range lane calls sqrt.
+    __device__ static inline float rof_sqrt_d(float s, double eps = 1.0e-8) {
+        const double x = (double)s + eps;
+        if (__builtin_amdgcn_ballot_w64(!(x < 0x1p+1000))) return (float)sqrt(x);   // also NaN
+        return (float)sqrt_dn(x);
+    }
     // (float)((double)num / (1.0 + lt)): PD_TV_core.c:191,249
     //
     // Without the double division: q = num * RN(1/c), c = 1 + lt, is within

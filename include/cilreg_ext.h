@@ -186,6 +186,15 @@ CILREG_API int cilreg_tv_energy_dev_f64(const double *d_U, const double *d_U0, d
  * values checked (2^32). */
 CILREG_API int cilreg_check_sqrt_rn(unsigned long long *mismatches, unsigned long long *checked);
 
+/* Self-check of the kernels' double square root of the 3D ROF / LLT-ROF flux
+ * normaliser, (float)sqrt((double)s + eps), which skips sqrt's scaling and
+ * special-value patches for in-range arguments: compares it with
+ * (float)sqrt((double)s + eps) on the device for every non-negative float bit
+ * pattern s (2^31 values, infinities and NaNs included) and returns the number
+ * of differing results (0 expected) and of values checked.  No counterpart in
+ * the reference. */
+CILREG_API int cilreg_check_rof_sqrt_d(double eps, unsigned long long *mismatches, unsigned long long *checked);
+
 /* Achievable HBM bandwidth of the current device, from streaming kernels
  * over arrays of `bytes` each (>= 1 MiB), the mean of `reps` timed launches:
  * gbps[0] copy (1 read + 1 write), gbps[1] read, gbps[2] write, gbps[3]

@@ -24,3 +24,20 @@ def test_sqrt_rn_equals_sqrtf_on_every_float():
     assert f(ctypes.byref(bad), ctypes.byref(seen)) == 0
     assert seen.value == 1 << 32
     assert bad.value == 0, f"{bad.value} floats whose sqrt_rn differs from sqrtf"
+
+
+@pytest.mark.parametrize("eps", [1e-8, 1e-12], ids=["rof", "llt"])
+def test_rof_sqrt_d_equals_double_sqrt_on_every_float(eps):
+    """The 3D D1 flux normaliser (float)sqrt((double)s + eps) of ROF (eps 1e-8)
+    and LLT-ROF (1e-12) skips sqrt's scaling and special-value patches for
+    in-range arguments; pinned to the device's sqrt for every non-negative
+    float s."""
+    from ccpi.filters.utils import require_gpu_lib
+    lib = require_gpu_lib()
+    f = lib.cilreg_check_rof_sqrt_d
+    f.argtypes = [ctypes.c_double] + [ctypes.POINTER(ctypes.c_ulonglong)] * 2
+    f.restype = ctypes.c_int
+    bad, seen = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    assert f(eps, ctypes.byref(bad), ctypes.byref(seen)) == 0
+    assert seen.value == 1 << 31
+    assert bad.value == 0, f"{bad.value} floats whose flux normaliser differs from the double sqrt"
