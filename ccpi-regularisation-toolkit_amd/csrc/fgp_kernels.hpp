@@ -285,6 +285,17 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
         cy[s] = in ? Y[s] : 0;
     }
     const bool own = X[0] < g.nx && Y[0] < g.ny;
+    // per slot: the clamped column index and the boundary tests the stages
+    // make, as bits (X, Y themselves are not kept)
+    constexpr int F_XGT0 = 1, F_YGT0 = 2, F_XLAST = 4, F_YLAST = 8, F_CXGT0 = 16, F_CYGT0 = 32;
+    I col[2];
+    int fl[2];
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        col[s] = (I)cy[s] * (I)g.nx + (I)cx[s];
+        fl[s] = (X[s] > 0 ? F_XGT0 : 0) | (Y[s] > 0 ? F_YGT0 : 0) | (X[s] == g.nx - 1 ? F_XLAST : 0) |
+                (Y[s] == g.ny - 1 ? F_YLAST : 0) | (cx[s] > 0 ? F_CXGT0 : 0) | (cy[s] > 0 ? F_CYGT0 : 0);
+    }
 
     __shared__ T Dk[2][BH + 1][BW + 1];    // D_k, planes by parity
     __shared__ T Dk1[2][BH + 1][BW + 1];   // D_{k+1}
@@ -308,7 +319,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
         #pragma unroll
         for (int s = 0; s < 2; ++s) {
             if (s < nslot) {
-                const I i = (I)(z0 - 2) * sxy + (I)cy[s] * nx + (I)cx[s];
+                const I i = (I)(z0 - 2) * sxy + col[s];
                 const T p = ldx<T>(a.P[2], i), q = ldx<T>(a.Pm[2], i);
                 rk[s][2] = p + m0 * (p - q);
             }
@@ -325,7 +336,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
             #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (s < nslot) {
-                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    const I i = (I)zi * sxy + col[s];
                     #pragma unroll
                     for (int c = 0; c < DIM; ++c) {
                         const T p = ldx<T>(a.P[c], i), q = ldx<T>(a.Pm[c], i);
@@ -345,24 +356,24 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
             #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (s < nslot) {
-                    const I i = (I)zi * sxy + (I)cy[s] * nx + (I)cx[s];
+                    const I i = (I)zi * sxy + col[s];
                     T b1r, b2r;
                     if (bx[s] > 0) {
                         b1r = Rk1[by[s]][bx[s] - 1];
                     } else {
-                        const I im = cx[s] > 0 ? i - 1 : i;
+                        const I im = (fl[s] & F_CXGT0) ? i - 1 : i;
                         const T b1p = ldx<T>(a.P[0], im), b1q = ldx<T>(a.Pm[0], im);
                         b1r = b1p + m0 * (b1p - b1q);
                     }
                     if (by[s] > 0) {
                         b2r = Rk2[by[s] - 1][bx[s]];
                     } else {
-                        const I jm = cy[s] > 0 ? i - nx : i;
+                        const I jm = (fl[s] & F_CYGT0) ? i - nx : i;
                         const T b2p = ldx<T>(a.P[1], jm), b2q = ldx<T>(a.Pm[1], jm);
                         b2r = b2p + m0 * (b2p - b2q);
                     }
-                    const T b1 = X[s] > 0 ? b1r : (T)0;
-                    const T b2 = Y[s] > 0 ? b2r : (T)0;
+                    const T b1 = (fl[s] & F_XGT0) ? b1r : (T)0;
+                    const T b2 = (fl[s] & F_YGT0) ? b2r : (T)0;
                     T d;
                     if (DIM == 3) {
                         const T b3 = zi > 0 ? rk[s][2] : (T)0;
@@ -373,7 +384,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     if (a.nonneg && d < (T)0) d = (T)0;
                     Dk[zi & 1][by[s]][bx[s]] = d;
                     if (s == 0 && own && zi >= z0 && zi < z1) {
-                        const I io = (I)zi * sxy + (I)Y[0] * nx + (I)X[0];
+                        const I io = (I)zi * sxy + col[0];
                         if (a.write_d0) stnt<T>(a.D0, io, d);
                         if (a.check0) {
                             const T df = d - ldx<T>(a.Dprev, io);
@@ -395,8 +406,8 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                 if (s < nslot) {
                     const int b = zc & 1;
                     const T d = Dk[b][by[s]][bx[s]];
-                    const T g1 = (X[s] == g.nx - 1) ? (T)0 : d - Dk[b][by[s]][bx[s] + 1];
-                    const T g2 = (Y[s] == g.ny - 1) ? (T)0 : d - Dk[b][by[s] + 1][bx[s]];
+                    const T g1 = (fl[s] & F_XLAST) ? (T)0 : d - Dk[b][by[s]][bx[s] + 1];
+                    const T g2 = (fl[s] & F_YLAST) ? (T)0 : d - Dk[b][by[s] + 1][bx[s]];
                     T p1 = rk[s][0] + step * g1;
                     T p2 = rk[s][1] + step * g2;
                     T p3 = 0;
@@ -406,7 +417,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     }
                     fgp_project<DIM, T, ISO>(p1, p2, p3);
                     if (s == 0 && own && zc >= z0 && zc < z1) {
-                        const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
+                        const I io = (I)zc * sxy + col[0];
                         stnt<T>(a.Pn[0], io, p1);
                         stnt<T>(a.Pn[1], io, p2);
                         if (DIM == 3) stnt<T>(a.Pn[2], io, p3);
@@ -433,8 +444,8 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                 if (s < nslot) {
                     const T b1r = R1s[by[s]][bx[s] > 0 ? bx[s] - 1 : 0];
                     const T b2r = R2s[by[s] > 0 ? by[s] - 1 : 0][bx[s]];
-                    const T b1 = X[s] > 0 ? b1r : (T)0;
-                    const T b2 = Y[s] > 0 ? b2r : (T)0;
+                    const T b1 = (fl[s] & F_XGT0) ? b1r : (T)0;
+                    const T b2 = (fl[s] & F_YGT0) ? b2r : (T)0;
                     T d;
                     if (DIM == 3) {
                         const T b3 = zc > 0 ? rq[s][2] : (T)0;
@@ -445,7 +456,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                     if (a.nonneg && d < (T)0) d = (T)0;
                     Dk1[zc & 1][by[s]][bx[s]] = d;
                     if (s == 0 && own && zc < z1) {
-                        const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
+                        const I io = (I)zc * sxy + col[0];
                         if (a.write_d1) stnt<T>(a.D1, io, d);
                         if (a.check1) {
                             const T df = d - Dk[zc & 1][by[0]][bx[0]];
@@ -462,8 +473,8 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
         if (zq >= z0 && zq < z1 && own) {
             const int b = zq & 1;
             const T d = Dk1[b][by[0]][bx[0]];
-            const T g1 = (X[0] == g.nx - 1) ? (T)0 : d - Dk1[b][by[0]][bx[0] + 1];
-            const T g2 = (Y[0] == g.ny - 1) ? (T)0 : d - Dk1[b][by[0] + 1][bx[0]];
+            const T g1 = (fl[0] & F_XLAST) ? (T)0 : d - Dk1[b][by[0]][bx[0] + 1];
+            const T g2 = (fl[0] & F_YLAST) ? (T)0 : d - Dk1[b][by[0] + 1][bx[0]];
             T p1 = rq[0][0] + step * g1;
             T p2 = rq[0][1] + step * g2;
             T p3 = 0;
@@ -472,7 +483,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                 p3 = rq[0][2] + step * g3;
             }
             fgp_project<DIM, T, ISO>(p1, p2, p3);
-            const I io = (I)zq * sxy + (I)Y[0] * nx + (I)X[0];
+            const I io = (I)zq * sxy + col[0];
             stnt<T>(a.Pnn[0], io, p1);
             stnt<T>(a.Pnn[1], io, p2);
             if (DIM == 3) stnt<T>(a.Pnn[2], io, p3);
