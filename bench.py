@@ -378,8 +378,8 @@ def batched_section(D, B, torch):
 
     short = [dict(sp, iters=2) for sp in specs]
     plain(short, "fp32")   # warm-up: workspaces, code objects
-    # the autotuner picks each job's z-chunk depth and pass width once per
-    # size class; tuning happens here, outside the timed runs
+    # the autotuner picks each job's pass width once per size class (grouped
+    # jobs march 32-plane chunks); tuning happens here, outside the timed runs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for prec in ("fp32", "bf16", "fp16"):

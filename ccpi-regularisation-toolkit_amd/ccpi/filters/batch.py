@@ -13,7 +13,7 @@ Method parameters: rof -- lambda, tau; fgp -- lambda, method, nonneg;
 pd -- lambda, L, method, nonneg; tgv -- lambda, alpha1, alpha0, L2.
 Every result is bit-identical to solving the volume alone with the same
 options (for a tolerance-stopped 3D job, the same ``zchunk``: the batch
-marches 16 planes per workgroup unless ``zchunk`` is given).
+marches 32 planes per workgroup unless ``zchunk`` is given).
 """
 import ctypes
 

@@ -5,7 +5,7 @@ The property under test is the one the engine is built on: a volume's result
 and infovector are bit-identical whether it is solved alone (the device API,
 itself pinned bit-exactly to the oracle by test_gpu_parity.py) or inside a
 batch / the server, in any mix of methods, dimensionalities, sizes, iteration
-counts, tolerances and admission times.  Single solves use zchunk=16, the
+counts, tolerances and admission times.  Single solves use zchunk=32, the
 batch's plane count, so tolerance-stopped jobs see the same per-block partial
 sums.
 """
@@ -49,7 +49,7 @@ def _spec(method, x, p):
 
 def _single(method, x, p, **kw):
     from ccpi.filters import device as D
-    kw.setdefault("zchunk", 16)
+    kw.setdefault("zchunk", 32)
     if method == "rof":
         return D.rof(x, p["lam"], p["iters"], p["tau"], p["tol"], **kw)
     if method == "fgp":
