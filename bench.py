@@ -584,7 +584,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=160)
+    ap.add_argument("--cpu-iters", type=int, default=240)
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the timed headline solves (for rocprofv3 passes)")
     ap.add_argument("--no-batch", action="store_true")
