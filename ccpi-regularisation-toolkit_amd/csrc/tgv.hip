@@ -44,11 +44,17 @@ static const void* tgv2_kernel_ptr(bool small) {
     }
 }
 
+// threads of the pass kernel tgv2_kernel_ptr returns
+template <int TY, typename T>
+static int tgv2_block_threads(bool small) {
+    return (std::is_same<T, float>::value && small) ? tgv2_threads<TY>() : TX * TY;
+}
+
 template <int TY, typename T, typename S>
 static void launch_tgv2(const Tgv2Args<T, S>& a, bool small, hipStream_t s) {
     dim3 grid(a.g.nblocks), block(TX, TY);
     if constexpr (std::is_same<T, float>::value) {
-        if (small) hipLaunchKernelGGL((tgv2_iter_kernel<TY, T, S>), grid, block, 0, s, a);
+        if (small) hipLaunchKernelGGL((tgv2_iter_kernel<TY, T, S>), grid, dim3(TX, tgv2_threads<TY>() / TX), 0, s, a);
         else hipLaunchKernelGGL((tgv2_iter_kernel_uncapped<TY, T, S, uint64_t>), grid, block, 0, s, a);
     } else {
         if (small) hipLaunchKernelGGL((tgv2_iter_kernel_uncapped<TY, T, S, uint32_t>), grid, block, 0, s, a);
@@ -83,8 +89,8 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     if (dim == 2)
         g = make_geom(nx, ny, nz, 4, 1);
     else if (width == 2)
-        g = ty == 8 ? plan_geom(tgv2_kernel_ptr<8, T, S>(small), TX * 8, nx, ny, nz, 8, opts.zchunk, 2.5)
-                    : plan_geom(tgv2_kernel_ptr<4, T, S>(small), TX * 4, nx, ny, nz, 4, opts.zchunk, 2.5);
+        g = ty == 8 ? plan_geom(tgv2_kernel_ptr<8, T, S>(small), tgv2_block_threads<8, T>(small), nx, ny, nz, 8, opts.zchunk, 2.5)
+                    : plan_geom(tgv2_kernel_ptr<4, T, S>(small), tgv2_block_threads<4, T>(small), nx, ny, nz, 4, opts.zchunk, 2.5);
     else if (ty == 8)
         g = plan_geom(small ? (const void*)tgv_iter_kernel<3, 8, T, S, uint32_t> : (const void*)tgv_iter_kernel<3, 8, T, S, uint64_t>,
                       TX * 8, nx, ny, nz, 8, opts.zchunk);

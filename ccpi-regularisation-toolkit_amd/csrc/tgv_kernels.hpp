@@ -347,34 +347,59 @@ struct Tgv2Shape {
     static constexpr int NT = TX * TY;
 };
 
-template <int TY, typename T, typename S, typename I>
+// Ring point h (0 <= h < 3 (TX+3) + 3 TY) of the region, in region coordinates.
+template <int TY>
+__device__ inline void tgv2_ring_point(int h, int& rx, int& ry) {
+    constexpr int BW = TX + 3;
+    if (h < 3 * BW) {
+        const int row = h / BW;
+        rx = h - row * BW;
+        ry = row < 2 ? row : TY + 2;           // rows y0-2, y0-1, y0+TY
+    } else {
+        const int k = h - 3 * BW, col = k / TY;
+        rx = col < 2 ? col : TX + 2;           // columns x0-2, x0-1, x0+TX
+        ry = k - col * TY + 2;
+    }
+}
+
+// RW = 0: TX x TY threads, each the own tile point (slot 0) and NR of them
+// also a ring point (slot 1, on the low or high waves by block index).
+// RW > 0: TX x TY + 64 RW threads, the first TX x TY the tile, the RW waves
+// after them the ring, one point per thread.
+template <int TY, typename T, typename S, typename I, int RW = 0>
 __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
     using Sh = Tgv2Shape<TY>;
     constexpr int BW = Sh::BW, BH = Sh::BH, NR = Sh::NR, NT = Sh::NT;
+    constexpr int NS = RW ? 1 : 2;   // slots per thread
+    static_assert(RW == 0 || 64 * RW >= NR, "ring waves must hold the ring");
     const Geom& g = a.g;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * TX + lx;
 
-    int bx[2], by[2];
-    bx[0] = lx + 2;
-    by[0] = ly + 2;
-    const int hbase = (tc.lin & 1) ? NT - NR : 0;
-    const int h = tid - hbase;
-    const bool ring = h >= 0 && h < NR;
-    {
+    int bx[2] = {0, 0}, by[2] = {0, 0};
+    int nslot;
+    bool tile0;   // slot 0 is a tile point
+    if (RW) {
+        tile0 = tid < NT;
+        const int h = tid - NT;
+        const bool ring = !tile0 && h < NR;
         int rx = 0, ry = 0;
-        if (h < 3 * BW) {
-            const int row = h / BW;
-            rx = h - row * BW;
-            ry = row < 2 ? row : TY + 2;           // rows y0-2, y0-1, y0+TY
-        } else {
-            const int k = h - 3 * BW, col = k / TY;
-            rx = col < 2 ? col : TX + 2;           // columns x0-2, x0-1, x0+TX
-            ry = k - col * TY + 2;
-        }
-        bx[1] = ring ? rx : 0;
-        by[1] = ring ? ry : 0;
+        if (ring) tgv2_ring_point<TY>(h, rx, ry);
+        bx[0] = tile0 ? lx + 2 : rx;
+        by[0] = tile0 ? ly + 2 : ry;
+        nslot = (tile0 || ring) ? 1 : 0;
+    } else {
+        tile0 = true;
+        bx[0] = lx + 2;
+        by[0] = ly + 2;
+        const int hbase = (tc.lin & 1) ? NT - NR : 0;
+        const int h = tid - hbase;
+        const bool ring = h >= 0 && h < NR;
+        int rx = 0, ry = 0;
+        if (ring) tgv2_ring_point<TY>(h, rx, ry);
+        bx[1] = rx;
+        by[1] = ry;
+        nslot = ring ? 2 : 1;
     }
-    const int nslot = ring ? 2 : 1;
     int X[2], Y[2], cx[2], cy[2];
     #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -384,7 +409,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         cx[s] = in ? X[s] : 0;
         cy[s] = in ? Y[s] : 0;
     }
-    const bool own = X[0] < g.nx && Y[0] < g.ny;
+    const bool own = tile0 && X[0] < g.nx && Y[0] < g.ny;
 
     __shared__ T PQ[L_N][BH + 1][BW + 1];      // P, Q channels read at x-1 / y-1
     __shared__ T UV[2][4][BH + 1][BW + 1];     // u_bar, v_bar of k+1, planes by parity
@@ -419,7 +444,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         // ---- S1: P, Q of k+1 at zi
         if (s12) {
             #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 if (s < nslot) {
                     tgv_dual_at<3, I>(in, cx[s], cy[s], zi, zi > zs, cu[s], cv[s], nu[s], nv[s], p1[s], q1[s]);
                     PQ[L_P1][by[s]][bx[s]] = p1[s][0];
@@ -437,7 +462,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         T fcur = 0;
         if (s12 && zi + 1 >= z0) {
             #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 if (s < nslot) {
                     const int xb = bx[s] > 0 ? bx[s] - 1 : 0, yb = by[s] > 0 ? by[s] - 1 : 0;
                     T xm[4], ym[4], zm[4];
@@ -480,7 +505,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         const bool s3 = zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
         if (s3) {
             #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 if (s < nslot) {
                     const int b = zc & 1;
                     const int xp = bx[s] + 1, yp = by[s] + 1;
@@ -556,7 +581,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         }
         if (s12) {
             #pragma unroll
-            for (int s = 0; s < 2; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 cu[s] = nu[s];
                 #pragma unroll
                 for (int c = 0; c < 3; ++c) { cv[s][c] = nv[s][c]; p1v[s][c] = p1[s][c]; }
@@ -583,13 +608,21 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
     }
 }
 
-// FP32 arithmetic, 32-bit offsets: registers capped at 128 so two 512-thread
-// workgroups fit a CU (the FP32-storage body needs about 122; BF16 / FP16
-// storage would take 129).
+// FP32 arithmetic, 32-bit offsets.  On 64 x 8 tiles the ring runs on four
+// waves of its own (768 threads, one point per thread) and registers are
+// capped at 80, so two workgroups fit a CU with six waves per SIMD; on 64 x 4
+// tiles each thread also carries a ring point and the cap is 128 (two
+// 256-thread workgroups per SIMD pair).
+template <int TY>
+constexpr int tgv2_ring_waves() { return TY == 8 ? 4 : 0; }
+template <int TY>
+constexpr int tgv2_threads() { return TX * TY + 64 * tgv2_ring_waves<TY>(); }
+
 template <int TY, typename T, typename S>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_iter_kernel(Tgv2Args<T, S> a) {
+__global__ __launch_bounds__(tgv2_threads<TY>()) __attribute__((amdgpu_waves_per_eu(tgv2_ring_waves<TY>() ? 6 : 4)))
+void tgv2_iter_kernel(Tgv2Args<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
-    tgv2_body<TY, T, S, uint32_t>(a, tile_of_block(a.g, TY));
+    tgv2_body<TY, T, S, uint32_t, tgv2_ring_waves<TY>()>(a, tile_of_block(a.g, TY));
 }
 
 // No register cap: 64-bit offsets (fields of 2 GiB and more) and FP64, whose
