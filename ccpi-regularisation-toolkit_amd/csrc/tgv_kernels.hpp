@@ -437,6 +437,29 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
     T zp2[4] = {0, 0, 0, 0};             // P3, Q13, Q23, Q33 of k+2 at the previous plane (slot 0)
     T f0 = 0;                            // f at the previous plane (slot 0)
     double sdd0 = 0.0, suu0 = 0.0, sdd1 = 0.0, suu1 = 0.0;
+    // With ring waves, values only tile threads use live in LDS slots of
+    // their own (thirteen registers fewer under the 80-register cap): the
+    // four residual sums, added to in the same order, and the slot-0 z
+    // pipeline (zp2, f0).
+    constexpr int NL = RW ? NT : 1;
+    __shared__ double acc[4][NL];
+    __shared__ T zl[5][NL];
+    const int lt = (RW && tile0) ? tid : 0;
+    if (RW && tile0) {
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c][lt] = 0.0;
+        #pragma unroll
+        for (int c = 0; c < 5; ++c) zl[c][lt] = (T)0;
+    }
+    auto add_acc = [&](int c0, double& r0, double& r1, double v0, double v1) {
+        if (RW) {
+            acc[c0][lt] += v0;
+            acc[c0 + 1][lt] += v1;
+        } else {
+            r0 += v0;
+            r1 += v1;
+        }
+    };
     const int zs = z0 >= 2 ? z0 - 2 : 0;
     for (int zi = zs; zi <= z1; ++zi) {
         const bool s12 = zi < nz;
@@ -487,8 +510,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                     if (s == 0 && own && zi >= z0 && zi < z1) {
                         if (a.check0) {
                             const T df = ubn - cu[0];
-                            sdd0 += (double)df * (double)df;
-                            suu0 += (double)ubn * (double)ubn;
+                            add_acc(0, sdd0, suu0, (double)df * (double)df, (double)ubn * (double)ubn);
                             if (a.Ualt) stnt<T>(a.Ualt, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], ubn);
                         }
                     }
@@ -554,12 +576,19 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
             #pragma unroll
             for (int c = 0; c < 3; ++c) vb[c] = UV[b][1 + c][by[0]][bx[0]];
             T ubn, vbn[3];
-            tgv_primal_math<3, T>(lam, tau, X[0], Y[0], zc, g.nx, g.ny, nz, p2[0], q2[0], xm, ym, zp2, ub, vb, f0,
+            T zpv[4], f0v = f0;
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) zpv[c] = zp2[c];
+            if (RW) {
+                #pragma unroll
+                for (int c = 0; c < 4; ++c) zpv[c] = zl[c][lt];
+                f0v = zl[4][lt];
+            }
+            tgv_primal_math<3, T>(lam, tau, X[0], Y[0], zc, g.nx, g.ny, nz, p2[0], q2[0], xm, ym, zpv, ub, vb, f0v,
                                   ubn, vbn);
             if (a.check1) {
                 const T df = ubn - ub;
-                sdd1 += (double)df * (double)df;
-                suu1 += (double)ubn * (double)ubn;
+                add_acc(2, sdd1, suu1, (double)df * (double)df, (double)ubn * (double)ubn);
             }
             const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
             stnt<T>(a.Uout, io, ubn);
@@ -574,10 +603,19 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         __syncthreads();
         // ---- shift the pipeline
         if (s3) {
-            zp2[0] = p2[0][2];
-            zp2[1] = q2[0][4];
-            zp2[2] = q2[0][5];
-            zp2[3] = q2[0][2];
+            if (RW) {
+                if (tile0) {
+                    zl[0][lt] = p2[0][2];
+                    zl[1][lt] = q2[0][4];
+                    zl[2][lt] = q2[0][5];
+                    zl[3][lt] = q2[0][2];
+                }
+            } else {
+                zp2[0] = p2[0][2];
+                zp2[1] = q2[0][4];
+                zp2[2] = q2[0][5];
+                zp2[3] = q2[0][2];
+            }
         }
         if (s12) {
             #pragma unroll
@@ -588,8 +626,18 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                 #pragma unroll
                 for (int c = 0; c < 6; ++c) q1v[s][c] = q1[s][c];
             }
-            f0 = fcur;
+            if (RW) {
+                if (tile0) zl[4][lt] = fcur;
+            } else {
+                f0 = fcur;
+            }
         }
+    }
+    if (RW && tile0) {
+        sdd0 = acc[0][lt];
+        suu0 = acc[1][lt];
+        sdd1 = acc[2][lt];
+        suu1 = acc[3][lt];
     }
     if (a.check0) {
         block_sum2(sdd0, suu0, red);
