@@ -326,6 +326,26 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
         }
     }
     double sdd0 = 0.0, suu0 = 0.0, sdd1 = 0.0, suu1 = 0.0;
+    // With 16-bit P the four per-thread residual sums live in LDS slots of
+    // their own between check planes (eight registers fewer: 256^3 BF16
+    // 0.103 -> 0.097 ms per iteration; FP32 measured 1% slower), added in
+    // the same order.
+    constexpr bool ACC_LDS = sizeof(S) == 2;
+    __shared__ double acc[4][ACC_LDS ? NT : 1];
+    const int at = ACC_LDS ? tid : 0;
+    if (ACC_LDS) {
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c][at] = 0.0;
+    }
+    auto add_acc = [&](int c0, double& r0, double& r1, double v0, double v1) {
+        if (ACC_LDS) {
+            acc[c0][at] += v0;
+            acc[c0 + 1][at] += v1;
+        } else {
+            r0 += v0;
+            r1 += v1;
+        }
+    };
     const int zstart = z0 >= 1 ? z0 - 1 : 0;
     const int zend = z1 + 1;
     for (int zi = zstart; zi <= zend; ++zi) {
@@ -388,8 +408,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                         if (a.write_d0) stnt<T>(a.D0, io, d);
                         if (a.check0) {
                             const T df = d - ldx<T>(a.Dprev, io);
-                            sdd0 += (double)df * (double)df;
-                            suu0 += (double)d * (double)d;
+                            add_acc(0, sdd0, suu0, (double)df * (double)df, (double)d * (double)d);
                         }
                     }
                 }
@@ -460,8 +479,7 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
                         if (a.write_d1) stnt<T>(a.D1, io, d);
                         if (a.check1) {
                             const T df = d - Dk[zc & 1][by[0]][bx[0]];
-                            sdd1 += (double)df * (double)df;
-                            suu1 += (double)d * (double)d;
+                            add_acc(2, sdd1, suu1, (double)df * (double)df, (double)d * (double)d);
                         }
                     }
                 }
@@ -498,6 +516,12 @@ __device__ inline void fgp2_body(const Fgp2Args<T, S>& a, const TileCoord& tc) {
             }
             if (s1) fz[s] = fn[s];
         }
+    }
+    if (ACC_LDS) {
+        sdd0 = acc[0][at];
+        suu0 = acc[1][at];
+        sdd1 = acc[2][at];
+        suu1 = acc[3][at];
     }
     if (a.check0) {
         block_sum2(sdd0, suu0, red);
