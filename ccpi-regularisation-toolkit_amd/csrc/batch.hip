@@ -569,7 +569,14 @@ __global__ __launch_bounds__(TX * TY) void pd2_persist_kernel(const PdJob<T, S>*
 
 // 3D TGV groups advance two levels per launch: pass q = k / 2 of a job reads
 // state set q & 1 and writes the other; a job's last iteration of an odd
-// count runs the one-iteration body on the same sets.
+// count runs the pass in its one-iteration mode on the same sets.  With
+// 32-bit offsets the pass runs its ring on four waves of its own (768
+// threads, 80 registers), as the single-volume pass does.
+template <int TY, typename I>
+constexpr int tgv2_group_rw() { return (TY == 8 && sizeof(I) == 4) ? 4 : 0; }
+template <int TY, typename I>
+constexpr int tgv2_group_threads() { return TX * TY + 64 * tgv2_group_rw<TY, I>(); }
+
 template <int TY, typename T, typename S, typename I>
 __device__ __attribute__((always_inline)) inline void tgv2_group_item(const TgvJob<T, S>* jobs, int njobs, int L, int level) {
     const TgvJob<T, S>* j;
@@ -578,73 +585,48 @@ __device__ __attribute__((always_inline)) inline void tgv2_group_item(const TgvJ
     if (k < 0) return;
     const int cur = (k >> 1) & 1, nxt = cur ^ 1;
     const TileCoord tc = tile_of_logical(uni_geom(j->h.g), TY, local);
-    if (k + 1 < j->h.iters) {
-        Tgv2Args<T, S> a;
-        a.f = uni(j->f);
-        a.U = uni(j->U[cur]);
-        a.Uout = uni(j->U[nxt]);
-        #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            a.V[c] = uni(j->V[cur][c]);
-            a.Vout[c] = uni(j->V[nxt][c]);
-            a.P[c] = uni(j->P[cur][c]);
-            a.Pout[c] = uni(j->P[nxt][c]);
-        }
-        #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            a.Q[c] = uni(j->Q[cur][c]);
-            a.Qout[c] = uni(j->Q[nxt][c]);
-        }
-        a.Ualt = uni(j->Ualt);
-        a.lambda = unf(j->lambda);
-        a.tau = unf(j->tau);
-        a.sigma = unf(j->tau);
-        a.alpha1 = unf(j->alpha1);
-        a.alpha0 = unf(j->alpha0);
-        a.check0 = job_checks(j->h, k);
-        a.check1 = job_checks(j->h, k + 1);
-        a.partials0 = uni(j->h.partials);
-        a.partials1 = uni(j->partials1);
-        a.ctrl = uni(j->h.ctrl);
-        a.g = uni_geom(j->h.g);
-        tgv2_body<TY, T, S, I>(a, tc);
-    } else {
-        TgvArgs<T, S> a;
-        a.f = uni(j->f);
-        a.U = uni(j->U[cur]);
-        a.Uout = uni(j->U[nxt]);
-        #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            a.V[c] = uni(j->V[cur][c]);
-            a.Vout[c] = uni(j->V[nxt][c]);
-            a.P[c] = uni(j->P[cur][c]);
-            a.Pout[c] = uni(j->P[nxt][c]);
-        }
-        #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            a.Q[c] = uni(j->Q[cur][c]);
-            a.Qout[c] = uni(j->Q[nxt][c]);
-        }
-        a.lambda = unf(j->lambda);
-        a.tau = unf(j->tau);
-        a.sigma = unf(j->tau);
-        a.alpha1 = unf(j->alpha1);
-        a.alpha0 = unf(j->alpha0);
-        a.check = job_checks(j->h, k);
-        a.partials = uni(j->h.partials);
-        a.ctrl = uni(j->h.ctrl);
-        a.g = uni_geom(j->h.g);
-        tgv_body<3, TY, T, S, I>(a, tc);
+    const bool two = k + 1 < j->h.iters;
+    Tgv2Args<T, S> a;
+    a.f = uni(j->f);
+    a.U = uni(j->U[cur]);
+    a.Uout = uni(j->U[nxt]);
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.V[c] = uni(j->V[cur][c]);
+        a.Vout[c] = uni(j->V[nxt][c]);
+        a.P[c] = uni(j->P[cur][c]);
+        a.Pout[c] = uni(j->P[nxt][c]);
     }
+    #pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        a.Q[c] = uni(j->Q[cur][c]);
+        a.Qout[c] = uni(j->Q[nxt][c]);
+    }
+    a.Ualt = uni(j->Ualt);
+    a.lambda = unf(j->lambda);
+    a.tau = unf(j->tau);
+    a.sigma = unf(j->tau);
+    a.alpha1 = unf(j->alpha1);
+    a.alpha0 = unf(j->alpha0);
+    a.check0 = job_checks(j->h, k);
+    a.check1 = two && job_checks(j->h, k + 1);
+    a.single = !two;
+    a.partials0 = uni(j->h.partials);
+    a.partials1 = uni(j->partials1);
+    a.ctrl = uni(j->h.ctrl);
+    a.g = uni_geom(j->h.g);
+    tgv2_body<TY, T, S, I, tgv2_group_rw<TY, I>()>(a, tc);
 }
 
 template <int TY, typename T, typename S, typename I>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total, int level) {
+__global__ __launch_bounds__((tgv2_group_threads<TY, I>())) __attribute__((amdgpu_waves_per_eu(tgv2_group_rw<TY, I>() ? 6 : 4)))
+void tgv2_group_kernel(const TgvJob<T, S>* jobs, int njobs, int total, int level) {
     tgv2_group_item<TY, T, S, I>(jobs, njobs, xcd_logical_block((int)blockIdx.x, total), level);
 }
 
 template <int TY, typename T, typename S, typename I>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4))) void tgv2_persist_kernel(const TgvJob<T, S>* jobs, int njobs, PersistQ* q) {
+__global__ __launch_bounds__((tgv2_group_threads<TY, I>())) __attribute__((amdgpu_waves_per_eu(tgv2_group_rw<TY, I>() ? 6 : 4)))
+void tgv2_persist_kernel(const TgvJob<T, S>* jobs, int njobs, PersistQ* q) {
     persistent_run(jobs, njobs, q, [&](int level, int L) { tgv2_group_item<TY, T, S, I>(jobs, njobs, L, level); });
 }
 
@@ -1516,9 +1498,11 @@ private:
                         if (w) go(tgv_group_kernel<3, TY, T, S, uint64_t>, tgv_persist_kernel<3, TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);
                         else go(tgv_group_kernel<3, TY, T, S, uint32_t>, tgv_persist_kernel<3, TY, T, S, uint32_t>, grid, block, st, jp, nact, total, level, q);
                     } else if (w) {
-                        go(tgv2_group_kernel<TY, T, S, uint64_t>, tgv2_persist_kernel<TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);
+                        go(tgv2_group_kernel<TY, T, S, uint64_t>, tgv2_persist_kernel<TY, T, S, uint64_t>, grid,
+                           dim3(TX, tgv2_group_threads<TY, uint64_t>() / TX), st, jp, nact, total, level, q);
                     } else {
-                        go(tgv2_group_kernel<TY, T, S, uint32_t>, tgv2_persist_kernel<TY, T, S, uint32_t>, grid, block, st, jp, nact, total, level, q);
+                        go(tgv2_group_kernel<TY, T, S, uint32_t>, tgv2_persist_kernel<TY, T, S, uint32_t>, grid,
+                           dim3(TX, tgv2_group_threads<TY, uint32_t>() / TX), st, jp, nact, total, level, q);
                     }
                 } else {
                     if (w) go(tgv_group_kernel<DIM, TY, T, S, uint64_t>, tgv_persist_kernel<DIM, TY, T, S, uint64_t>, grid, block, st, jp, nact, total, level, q);

@@ -196,6 +196,7 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
             a.alpha0 = alpha0;
             a.check0 = c0;
             a.check1 = c1;
+            a.single = 0;
             a.partials0 = partials0;
             a.partials1 = partials1;
             a.ctrl = ctrl;

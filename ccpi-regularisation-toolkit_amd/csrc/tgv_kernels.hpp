@@ -320,6 +320,8 @@ __global__ __launch_bounds__(TX * TY) void tgv_iter_kernel(TgvArgs<T, S> a) {
 //   S2  u_bar, v_bar of k+1 at zi (P, Q of k+1 at zi, x-1, y-1 and zi-1)
 //   S3  P, Q of k+2 at zi-1      (u_bar, v_bar of k+1 at zi-1, +x, +y, zi)
 //   S4  u_bar, v_bar of k+2 at zi-1 on T; all 13 state fields stored.
+// In the one-iteration mode (single) S2 stores the state of k+1 on T, as the
+// one-iteration kernel does, and S3 / S4 do nothing.
 template <typename T, typename S>
 struct Tgv2Args {
     const T* f;
@@ -334,6 +336,7 @@ struct Tgv2Args {
     T* Ualt;            // u_bar after iteration k (written when check0 and non-null: the output of a stop at k)
     T lambda, tau, sigma, alpha1, alpha0;
     int check0, check1;
+    int single;         // advance one iteration (k -> k+1) only: S2 stores the state
     double* partials0;
     double* partials1;
     const Ctrl* ctrl;
@@ -508,6 +511,17 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                     tgv_primal_math<3, T>(lam, tau, X[s], Y[s], zi, g.nx, g.ny, nz, p1[s], q1[s], xm, ym, zm,
                                           cu[s], cv[s], fv, ubn, vbn);
                     if (s == 0 && own && zi >= z0 && zi < z1) {
+                        if (a.single) {   // the state of k+1, as tgv_body stores it
+                            const I io = (I)zi * sxy + (I)Y[0] * nx + (I)X[0];
+                            stnt<T>(a.Uout, io, ubn);
+                            #pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                stnt<T>(a.Vout[c], io, vbn[c]);
+                                stnt<T>(a.Pout[c], io, p1[0][c]);
+                            }
+                            #pragma unroll
+                            for (int c = 0; c < 6; ++c) stnt<T>(a.Qout[c], io, q1[0][c]);
+                        }
                         if (a.check0) {
                             const T df = ubn - cu[0];
                             add_acc(0, sdd0, suu0, (double)df * (double)df, (double)ubn * (double)ubn);
@@ -524,7 +538,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         // ---- S3: P, Q of k+2 at zc
         const int zc = zi - 1;
         T p2[2][3] = {}, q2[2][6] = {};
-        const bool s3 = zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
+        const bool s3 = !a.single && zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
         if (s3) {
             #pragma unroll
             for (int s = 0; s < NS; ++s) {
