@@ -368,8 +368,10 @@ __device__ inline void tgv2_ring_point(int h, int& rx, int& ry) {
 // RW = 0: TX x TY threads, each the own tile point (slot 0) and NR of them
 // also a ring point (slot 1, on the low or high waves by block index).
 // RW > 0: TX x TY + 64 RW threads, the first TX x TY the tile, the RW waves
-// after them the ring, one point per thread.
-template <int TY, typename T, typename S, typename I, int RW = 0>
+// after them the ring, one point per thread.  SINGLE: whether the
+// one-iteration mode is compiled in (the grouped kernel's odd last
+// iterations; the single-volume solver runs those on tgv_body).
+template <int TY, typename T, typename S, typename I, int RW = 0, bool SINGLE = true>
 __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
     using Sh = Tgv2Shape<TY>;
     constexpr int BW = Sh::BW, BH = Sh::BH, NR = Sh::NR, NT = Sh::NT;
@@ -511,7 +513,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                     tgv_primal_math<3, T>(lam, tau, X[s], Y[s], zi, g.nx, g.ny, nz, p1[s], q1[s], xm, ym, zm,
                                           cu[s], cv[s], fv, ubn, vbn);
                     if (s == 0 && own && zi >= z0 && zi < z1) {
-                        if (a.single) {   // the state of k+1, as tgv_body stores it
+                        if (SINGLE && a.single) {   // the state of k+1, as tgv_body stores it
                             const I io = (I)zi * sxy + (I)Y[0] * nx + (I)X[0];
                             stnt<T>(a.Uout, io, ubn);
                             #pragma unroll
@@ -538,7 +540,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
         // ---- S3: P, Q of k+2 at zc
         const int zc = zi - 1;
         T p2[2][3] = {}, q2[2][6] = {};
-        const bool s3 = !a.single && zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
+        const bool s3 = !(SINGLE && a.single) && zc >= (z0 >= 1 ? z0 - 1 : 0) && zc <= z1 - 1;
         if (s3) {
             #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -684,7 +686,7 @@ template <int TY, typename T, typename S>
 __global__ __launch_bounds__(tgv2_threads<TY>()) __attribute__((amdgpu_waves_per_eu(tgv2_ring_waves<TY>() ? 6 : 4)))
 void tgv2_iter_kernel(Tgv2Args<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
-    tgv2_body<TY, T, S, uint32_t, tgv2_ring_waves<TY>()>(a, tile_of_block(a.g, TY));
+    tgv2_body<TY, T, S, uint32_t, tgv2_ring_waves<TY>(), false>(a, tile_of_block(a.g, TY));
 }
 
 // No register cap: 64-bit offsets (fields of 2 GiB and more) and FP64, whose
@@ -692,7 +694,7 @@ void tgv2_iter_kernel(Tgv2Args<T, S> a) {
 template <int TY, typename T, typename S, typename I>
 __global__ __launch_bounds__(TX * TY) void tgv2_iter_kernel_uncapped(Tgv2Args<T, S> a) {
     if (solve_stopped(a.ctrl)) return;
-    tgv2_body<TY, T, S, I>(a, tile_of_block(a.g, TY));
+    tgv2_body<TY, T, S, I, 0, false>(a, tile_of_block(a.g, TY));
 }
 
 }  // namespace cilreg
