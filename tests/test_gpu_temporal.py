@@ -68,6 +68,21 @@ def test_depth2_matches_depth1_tolerance_stops(check_every):
     assert stopped
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_depth2_matches_depth1_tolerance_stops_16bit(precision):
+    """With 16-bit P the pass keeps its FP64 residual sums in LDS slots: the
+    reported residual and the stop iteration must equal the one-iteration
+    kernel's, 3D and 2D."""
+    for shape, seed in (((16, 48, 80), 13), ((200, 240), 17)):
+        x = _vol(shape, seed)
+        stopped = False
+        for tol in (1e-2, 3e-3, 1e-4):
+            a, b = _pair(x, lam=0.03, iters=120, tol=tol, check_every=1, precision=precision)
+            _same(a, b, f"{shape} {precision} tol={tol}")
+            stopped |= int(a[1][0]) < 120
+        assert stopped, f"{shape} {precision}: no tolerance stop exercised"
+
+
 def test_depth2_wide_index():
     x = _vol((12, 30, 66), 14)
     a, b = _pair(x, lam=0.03, iters=9, wide_index=True)
@@ -156,6 +171,21 @@ def test_tgv_depth2_matches_depth1_tolerance_stops(check_every):
             if int(a[1][0]) < 150:
                 stopped.add(int(a[1][0]) % 2)
     assert stopped, "no tolerance stop exercised"
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_tgv_depth2_matches_depth1_tolerance_stops_16bit(precision):
+    """16-bit duals on the ring-wave pass (residual sums and the tile point's
+    z pipeline in LDS slots): stop iteration, reported residual and output
+    equal the one-iteration kernel's."""
+    x = _vol((16, 48, 80), 23)
+    stopped = False
+    for tol in (3e-2, 1e-2, 3e-3):
+        a, b = _tgv_pair(x, lam=0.03, alpha1=1.0, alpha0=2.0, iters=150, tol=tol, check_every=1,
+                         precision=precision)
+        _same(a, b, f"TGV {precision} tol={tol}")
+        stopped |= int(a[1][0]) < 150
+    assert stopped, f"TGV {precision}: no tolerance stop exercised"
 
 
 def test_tgv_depth2_wide_index():
