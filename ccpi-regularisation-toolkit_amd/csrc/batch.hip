@@ -1131,7 +1131,16 @@ private:
                 break;
             default:                                                                       // U x2, (V, P, Q) x2
                 res(nT); res(nT);
-                for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
+                if (tgv_pairs<3, S>() && d == 3) {
+                    // pair fields (tgv_pairs): channels 2k and 2k + 1 of a set share one buffer of 2 n values
+                    for (int k = 0; k < 2 * 6; ++k) {
+                        const size_t o = ar.reserve(2 * nS);
+                        j.off_fld[m++] = o;
+                        j.off_fld[m++] = o;
+                    }
+                } else {
+                    for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
+                }
                 if (j.width == 2 && j.pub.epsil != 0.0f) res(nT);   // Ualt of two-iteration passes
                 break;
         }
@@ -1204,7 +1213,11 @@ private:
                 break;
             default:
                 launch_copy(j.fld[0], j.f, nT, s_);
-                for (int i = 0; i < 2 * d + nq; ++i) launch_zero(j.fld[2 + i], nS, s_);   // set 0 of V, P, Q
+                if (tgv_pairs<3, S>() && d == 3) {   // set 0 of V, P, Q: six pair fields
+                    for (int i = 0; i < 2 * d + nq; i += 2) launch_zero(j.fld[2 + i], 2 * nS, s_);
+                } else {
+                    for (int i = 0; i < 2 * d + nq; ++i) launch_zero(j.fld[2 + i], nS, s_);
+                }
                 break;
         }
         return 0;

@@ -133,6 +133,40 @@ __device__ inline void stnt(S* p, I e, T v) {
     }
 }
 
+// Pair fields: two values of one point side by side (elements 2e and 2e + 1),
+// moved by one access of 2 sizeof(S) bytes.  The host keeps 2 n sizeof(S)
+// within the 32-bit offset range when I = uint32_t.
+template <typename S> struct PairV { typedef S v __attribute__((ext_vector_type(2))); };
+template <> struct PairV<bf16_t> { typedef uint32_t v; };
+
+template <typename T, typename I, typename S>
+__device__ inline void ldpair(const S* p, I e, T& lo, T& hi) {
+    using V = typename PairV<S>::v;
+    const V w = *gmem(IdxT<I>::at((const V*)p, e));
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        lo = (T)__uint_as_float(w << 16);
+        hi = (T)__uint_as_float(w & 0xffff0000u);
+    } else {
+        lo = (T)w.x;
+        hi = (T)w.y;
+    }
+}
+template <typename T, typename I, typename S>
+__device__ inline void stpair_nt(S* p, I e, T lo, T hi) {
+    using V = typename PairV<S>::v;
+    V w;
+    if constexpr (std::is_same<S, bf16_t>::value) {
+        w = (uint32_t)Store<bf16_t>::bits_of((float)lo) | ((uint32_t)Store<bf16_t>::bits_of((float)hi) << 16);
+    } else if constexpr (std::is_same<S, _Float16>::value) {
+        w.x = (_Float16)(float)lo;
+        w.y = (_Float16)(float)hi;
+    } else {
+        w.x = (S)lo;
+        w.y = (S)hi;
+    }
+    __builtin_nontemporal_store(w, gmem(IdxT<I>::at((V*)p, e)));
+}
+
 // Largest element count addressed with 32-bit byte offsets for element size es.
 inline bool fits_u32(size_t n_elems, size_t es) { return n_elems * es < ((size_t)1 << 31); }
 

@@ -15,30 +15,31 @@ struct WarmState {
     int warm;         // 1: start from d_state (and tstate); 0: cold start, state only written
 };
 
+// stride: element distance in the working buffer (2 for one half of a pair field)
 template <typename S>
-__global__ __launch_bounds__(256) void state_in_kernel(S* d, const float* s, size_t n) {
+__global__ __launch_bounds__(256) void state_in_kernel(S* d, const float* s, size_t n, int stride) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-        Store<S>::st(d, i, s[i]);
+        Store<S>::st(d, i * stride, s[i]);
 }
 template <typename S>
-__global__ __launch_bounds__(256) void state_out_kernel(float* d, const S* s, size_t n) {
+__global__ __launch_bounds__(256) void state_out_kernel(float* d, const S* s, size_t n, int stride) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-        d[i] = (float)Store<S>::ld(s, i);
+        d[i] = (float)Store<S>::ld(s, i * stride);
 }
 
 inline dim3 state_grid(size_t n) {
     size_t b = (n + 255) / 256;
     return dim3((unsigned)(b > 16384 ? 16384 : (b ? b : 1)));
 }
-// FP32 state channel -> working buffer in storage type S
+// FP32 state channel -> working buffer in storage type S (every stride-th element)
 template <typename S>
-void state_in(S* dst, const float* src, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(state_in_kernel<S>, state_grid(n), dim3(256), 0, s, dst, src, n);
+void state_in(S* dst, const float* src, size_t n, hipStream_t s, int stride = 1) {
+    hipLaunchKernelGGL(state_in_kernel<S>, state_grid(n), dim3(256), 0, s, dst, src, n, stride);
 }
-// working buffer in storage type S -> FP32 state channel
+// working buffer in storage type S (every stride-th element) -> FP32 state channel
 template <typename S>
-void state_out(float* dst, const S* src, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(state_out_kernel<S>, state_grid(n), dim3(256), 0, s, dst, src, n);
+void state_out(float* dst, const S* src, size_t n, hipStream_t s, int stride = 1) {
+    hipLaunchKernelGGL(state_out_kernel<S>, state_grid(n), dim3(256), 0, s, dst, src, n, stride);
 }
 
 }  // namespace cilreg

@@ -81,7 +81,10 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         return 0;
     }
     const int nq = dim == 3 ? 6 : 3;
-    const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > sizeof(S) ? sizeof(T) : sizeof(S));
+    // 16-bit state in 3D lives in pair fields (two channels per 32-bit word, tgv_pairs)
+    const bool pairs = tgv_pairs<3, S>() && dim == 3;
+    const size_t pes = pairs ? 2 * sizeof(S) : sizeof(S);   // bytes per point of one field
+    const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > pes ? sizeof(T) : pes);
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
     // returning the state after a stop needs the stop iteration's own pass
     const int width = (dim == 3 && opts.depth != 1 && iter > 1 && !(ws && epsil != (T)0)) ? 2 : 1;
@@ -102,28 +105,58 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
     for (int b = 0; b < 2; ++b) {
         oU[b] = ar.reserve(n * sizeof(T));
-        for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
-        for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
-        for (int c = 0; c < nq; ++c) oQ[b][c] = ar.reserve(n * sizeof(S));
+        if (pairs) {   // six pair fields; the second channel of a pair names the same buffer
+            const size_t pb = 2 * n * sizeof(S);
+            oV[b][0] = oV[b][1] = ar.reserve(pb);    // V1, V2
+            oV[b][2] = oP[b][0] = ar.reserve(pb);    // V3, P1
+            oP[b][1] = oP[b][2] = ar.reserve(pb);    // P2, P3
+            for (int c = 0; c < 6; c += 2) oQ[b][c] = oQ[b][c + 1] = ar.reserve(pb);
+        } else {
+            for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
+            for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
+            for (int c = 0; c < nq; ++c) oQ[b][c] = ar.reserve(n * sizeof(S));
+        }
     }
     oCtrl = ar.reserve(sizeof(Ctrl));
     oPart = ar.reserve((size_t)g.nblocks * 4 * sizeof(double));
     if (ar.allocate() != 0) return -1;
+    // State channel ch (V1..3, P1..3, then Q) of set b: its first element and
+    // element stride.
+    auto chan = [&](int b, int ch, int& stride) -> S* {
+        if (pairs) {
+            const size_t pf[6] = {oV[b][0], oV[b][2], oP[b][1], oQ[b][0], oQ[b][2], oQ[b][4]};
+            stride = 2;
+            return (S*)ar.at(pf[ch / 2]) + (ch & 1);
+        }
+        stride = 1;
+        return (S*)ar.at(ch < 3 ? oV[b][ch] : ch < 6 ? oP[b][ch - 3] : oQ[b][ch - 6]);
+    };
+    const int nch = 2 * dim + nq;   // channels in state order (3D: 12, 2D: 7)
+    auto chan_of = [&](int k) { return k < dim ? k : k < 2 * dim ? 3 + (k - dim) : 6 + (k - 2 * dim); };
     launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
-    for (int c = 0; c < dim; ++c) {
-        launch_zero(ar.at(oV[0][c]), n * sizeof(S), s);
-        launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
+    if (pairs) {
+        for (int k = 0; k < 6; ++k) {
+            int st;
+            launch_zero(chan(0, 2 * k, st), 2 * n * sizeof(S), s);
+        }
+    } else {
+        for (int c = 0; c < dim; ++c) {
+            launch_zero(ar.at(oV[0][c]), n * sizeof(S), s);
+            launch_zero(ar.at(oP[0][c]), n * sizeof(S), s);
+        }
+        for (int c = 0; c < nq; ++c) launch_zero(ar.at(oQ[0][c]), n * sizeof(S), s);
     }
-    for (int c = 0; c < nq; ++c) launch_zero(ar.at(oQ[0][c]), n * sizeof(S), s);
     // caller state channels: u_bar, V1..3, P1..3, Q (Q11, Q22, Q33, Q12, Q13, Q23 in 3D; Q11, Q22, Q12 in 2D)
     if constexpr (std::is_same<T, float>::value) {
         if (ws && ws->warm) {
             state_in<T>((T*)ar.at(oU[0]), ws->d_state, n, s);
-            for (int c = 0; c < dim; ++c) {
-                state_in<S>((S*)ar.at(oV[0][c]), ws->d_state + (size_t)(1 + c) * n, n, s);
-                state_in<S>((S*)ar.at(oP[0][c]), ws->d_state + (size_t)(4 + c) * n, n, s);
+            for (int k = 0; k < nch; ++k) {
+                const int ch = chan_of(k);
+                const size_t at = ch < 3 ? 1 + ch : ch < 6 ? 4 + (ch - 3) : 7 + (ch - 6);
+                int st;
+                S* p = chan(0, ch, st);
+                state_in<S>(p, ws->d_state + at * n, n, s, st);
             }
-            for (int c = 0; c < nq; ++c) state_in<S>((S*)ar.at(oQ[0][c]), ws->d_state + (size_t)(7 + c) * n, n, s);
         }
     }
     Ctrl* ctrl = (Ctrl*)ar.at(oCtrl);
@@ -236,12 +269,13 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
                 if (last >= pass_k[q] && last < pass_k[q] + pass_n[q]) set = pass_out[q];
             if (set < 0) { set_error("TGV: no pass covers iteration %d", last); return -1; }
             state_out<T>(ws->d_state, (const T*)ar.at(oU[set]), n, s);
-            for (int c = 0; c < dim; ++c) {
-                state_out<S>(ws->d_state + (size_t)(1 + c) * n, (const S*)ar.at(oV[set][c]), n, s);
-                state_out<S>(ws->d_state + (size_t)(4 + c) * n, (const S*)ar.at(oP[set][c]), n, s);
+            for (int k = 0; k < nch; ++k) {
+                const int ch = chan_of(k);
+                const size_t at = ch < 3 ? 1 + ch : ch < 6 ? 4 + (ch - 3) : 7 + (ch - 6);
+                int st;
+                const S* p = chan(set, ch, st);
+                state_out<S>(ws->d_state + at * n, p, n, s, st);
             }
-            for (int c = 0; c < nq; ++c)
-                state_out<S>(ws->d_state + (size_t)(7 + c) * n, (const S*)ar.at(oQ[set][c]), n, s);
         }
     }
     CILREG_CK(hipStreamSynchronize(s));

@@ -93,6 +93,16 @@ __device__ inline void tgv_dual_math(T sigma, T alpha1, T alpha0, bool lx, bool 
     }
 }
 
+// 3D state in 16-bit storage keeps two components of a point in one 32-bit
+// word (pair fields, the first component at the even element): (V1, V2) in
+// V[0], (V3, P1) in V[2], (P2, P3) in P[1], (Q11, Q22) in Q[0], (Q33, Q12) in
+// Q[2], (Q13, Q23) in Q[4]; the other component pointers are unused.  A point
+// then loads 11 words instead of 18 half-words and stores 6 instead of 12
+// (192x512x512 BF16: 0.74 -> 0.66 ms per iteration, and the ring-wave pass
+// no longer spills).
+template <int DIM, typename S>
+constexpr bool tgv_pairs() { return DIM == 3 && sizeof(S) == 2; }
+
 // Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
 // the point) are passed in when the caller carries them from the previous
 // plane (carried = true) and loaded otherwise; uz / vz return u_bar and v_bar
@@ -107,6 +117,31 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
     const I i = (I)z * sxy + (I)py * nx + (I)px;
     const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
     const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
+    if constexpr (tgv_pairs<DIM, S>()) {
+        T v3i, p1i, unused;
+        ldpair<T>(a.V[2], i, v3i, p1i);
+        if (!carried) {
+            u = ldx<T>(a.U, i);
+            ldpair<T>(a.V[0], i, v[0], v[1]);
+            v[2] = v3i;
+        }
+        const T ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
+        uz = ldx<T>(a.U, iz);
+        T vx[3], vy[3], po[3], qo[6];
+        ldpair<T>(a.V[0], ix, vx[0], vx[1]);
+        ldpair<T>(a.V[2], ix, vx[2], unused);
+        ldpair<T>(a.V[0], iy, vy[0], vy[1]);
+        ldpair<T>(a.V[2], iy, vy[2], unused);
+        ldpair<T>(a.V[0], iz, vz[0], vz[1]);
+        ldpair<T>(a.V[2], iz, vz[2], unused);
+        po[0] = p1i;
+        ldpair<T>(a.P[1], i, po[1], po[2]);
+        ldpair<T>(a.Q[0], i, qo[0], qo[1]);
+        ldpair<T>(a.Q[2], i, qo[2], qo[3]);
+        ldpair<T>(a.Q[4], i, qo[4], qo[5]);
+        tgv_dual_math<DIM, T>(a.sigma, a.alpha1, a.alpha0, lx, ly, lz, u, v, ux, uy, uz, vx, vy, vz, po, qo, p, q);
+        return;
+    }
     if (!carried) {
         u = ldx<T>(a.U, i);
         #pragma unroll
@@ -126,6 +161,29 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
     #pragma unroll
     for (int c = 0; c < nq; ++c) qo[c] = ldx<T>(a.Q[c], i);
     tgv_dual_math<DIM, T>(a.sigma, a.alpha1, a.alpha0, lx, ly, lz, u, v, ux, uy, uz, vx, vy, vz, po, qo, p, q);
+}
+
+// Stores the state of one point (u_bar, v_bar, P, Q) at element i.
+template <int DIM, typename I, typename T, typename S, class A>
+__device__ inline void tgv_store_state(const A& a, I i, T ubn, const T vbn[3], const T p[3], const T q[6]) {
+    stnt<T>(a.Uout, i, ubn);
+    if constexpr (tgv_pairs<DIM, S>()) {
+        stpair_nt<T>(a.Vout[0], i, vbn[0], vbn[1]);
+        stpair_nt<T>(a.Vout[2], i, vbn[2], p[0]);
+        stpair_nt<T>(a.Pout[1], i, p[1], p[2]);
+        stpair_nt<T>(a.Qout[0], i, q[0], q[1]);
+        stpair_nt<T>(a.Qout[2], i, q[2], q[3]);
+        stpair_nt<T>(a.Qout[4], i, q[4], q[5]);
+    } else {
+        #pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            stnt<T>(a.Vout[c], i, vbn[c]);
+            stnt<T>(a.Pout[c], i, p[c]);
+        }
+        const int nq = DIM == 3 ? 6 : 3;
+        #pragma unroll
+        for (int c = 0; c < nq; ++c) stnt<T>(a.Qout[c], i, q[c]);
+    }
 }
 
 // adjoint difference: own value at index 0, minus the predecessor at the last
@@ -265,15 +323,7 @@ __device__ inline void tgv_body(const TgvArgs<T, S>& a, const TileCoord& tc) {
                 sdd += (double)df * (double)df;
                 suu += (double)ubn * (double)ubn;
             }
-            stnt<T>(a.Uout, i, ubn);
-            #pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                stnt<T>(a.Vout[c], i, vbn[c]);
-                stnt<T>(a.Pout[c], i, p[c]);
-            }
-            const int nq = DIM == 3 ? 6 : 3;
-            #pragma unroll
-            for (int c = 0; c < nq; ++c) stnt<T>(a.Qout[c], i, q[c]);
+            tgv_store_state<DIM, I, T, S>(a, i, ubn, vbn, p, q);
         }
         __syncthreads();
         zp3 = p[2]; zq33 = q[2]; zq13 = q[4]; zq23 = q[5];
@@ -513,17 +563,9 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                     tgv_primal_math<3, T>(lam, tau, X[s], Y[s], zi, g.nx, g.ny, nz, p1[s], q1[s], xm, ym, zm,
                                           cu[s], cv[s], fv, ubn, vbn);
                     if (s == 0 && own && zi >= z0 && zi < z1) {
-                        if (SINGLE && a.single) {   // the state of k+1, as tgv_body stores it
-                            const I io = (I)zi * sxy + (I)Y[0] * nx + (I)X[0];
-                            stnt<T>(a.Uout, io, ubn);
-                            #pragma unroll
-                            for (int c = 0; c < 3; ++c) {
-                                stnt<T>(a.Vout[c], io, vbn[c]);
-                                stnt<T>(a.Pout[c], io, p1[0][c]);
-                            }
-                            #pragma unroll
-                            for (int c = 0; c < 6; ++c) stnt<T>(a.Qout[c], io, q1[0][c]);
-                        }
+                        if (SINGLE && a.single)   // the state of k+1, as tgv_body stores it
+                            tgv_store_state<3, I, T, S>(a, (I)zi * sxy + (I)Y[0] * nx + (I)X[0], ubn, vbn, p1[0],
+                                                        q1[0]);
                         if (a.check0) {
                             const T df = ubn - cu[0];
                             add_acc(0, sdd0, suu0, (double)df * (double)df, (double)ubn * (double)ubn);
@@ -607,14 +649,7 @@ __device__ inline void tgv2_body(const Tgv2Args<T, S>& a, const TileCoord& tc) {
                 add_acc(2, sdd1, suu1, (double)df * (double)df, (double)ubn * (double)ubn);
             }
             const I io = (I)zc * sxy + (I)Y[0] * nx + (I)X[0];
-            stnt<T>(a.Uout, io, ubn);
-            #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                stnt<T>(a.Vout[c], io, vbn[c]);
-                stnt<T>(a.Pout[c], io, p2[0][c]);
-            }
-            #pragma unroll
-            for (int c = 0; c < 6; ++c) stnt<T>(a.Qout[c], io, q2[0][c]);
+            tgv_store_state<3, I, T, S>(a, io, ubn, vbn, p2[0], q2[0]);
         }
         __syncthreads();
         // ---- shift the pipeline
