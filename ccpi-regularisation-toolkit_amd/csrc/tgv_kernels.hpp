@@ -98,7 +98,7 @@ __device__ inline void tgv_dual_math(T sigma, T alpha1, T alpha0, bool lx, bool 
 // V[0], (V3, P1) in V[2], (P2, P3) in P[1], (Q11, Q22) in Q[0], (Q33, Q12) in
 // Q[2], (Q13, Q23) in Q[4]; the other component pointers are unused.  A point
 // then loads 11 words instead of 18 half-words and stores 6 instead of 12
-// (192x512x512 BF16: 0.74 -> 0.66 ms per iteration, and the ring-wave pass
+// (192x512x512 BF16: 0.71 -> 0.62 ms per iteration, and the ring-wave pass
 // no longer spills).
 template <int DIM, typename S>
 constexpr bool tgv_pairs() { return DIM == 3 && sizeof(S) == 2; }
