@@ -810,6 +810,9 @@ public:
         j->nz = j->dim == 2 ? 1 : pj.dimZ;
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
+        // 3D TGV state in pair fields spans 2 n values of S per field
+        if (j->pub.method == CILREG_TGV && j->dim == 3 && tgv_pairs<3, S>() && !fits_u32(j->n, 2 * sizeof(S)))
+            j->wide = true;
         j->ty = j->dim == 3 ? 8 : 4;
         // two-iteration passes: FGP, PD and TGV in 3D, FGP on small 2D images
         const bool two = (j->pub.method != CILREG_ROF && j->dim == 3) ||

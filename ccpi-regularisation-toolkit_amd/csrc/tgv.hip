@@ -81,7 +81,7 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         return 0;
     }
     const int nq = dim == 3 ? 6 : 3;
-    // 16-bit state in 3D lives in pair fields (two channels per 32-bit word, tgv_pairs)
+    // 3D state in FP32 or 16-bit storage lives in pair fields (two channels side by side, tgv_pairs)
     const bool pairs = tgv_pairs<3, S>() && dim == 3;
     const size_t pes = pairs ? 2 * sizeof(S) : sizeof(S);   // bytes per point of one field
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > pes ? sizeof(T) : pes);

@@ -93,15 +93,15 @@ __device__ inline void tgv_dual_math(T sigma, T alpha1, T alpha0, bool lx, bool 
     }
 }
 
-// 3D state in 16-bit storage keeps two components of a point in one 32-bit
-// word (pair fields, the first component at the even element): (V1, V2) in
-// V[0], (V3, P1) in V[2], (P2, P3) in P[1], (Q11, Q22) in Q[0], (Q33, Q12) in
-// Q[2], (Q13, Q23) in Q[4]; the other component pointers are unused.  A point
-// then loads 11 words instead of 18 half-words and stores 6 instead of 12
-// (192x512x512 BF16: 0.71 -> 0.62 ms per iteration, and the ring-wave pass
-// no longer spills).
+// 3D state in FP32 or 16-bit storage keeps two components of a point side
+// by side (pair fields, the first component at the even element): (V1, V2)
+// in V[0], (V3, P1) in V[2], (P2, P3) in P[1], (Q11, Q22) in Q[0], (Q33, Q12)
+// in Q[2], (Q13, Q23) in Q[4]; the other component pointers are unused.  A
+// point then issues 11 loads of a pair instead of 18 single loads and 6
+// stores instead of 12 (192x512x512: FP32 0.87 -> 0.79 ms, BF16 0.71 -> 0.62
+// ms per iteration).  FP64 state keeps one field per component.
 template <int DIM, typename S>
-constexpr bool tgv_pairs() { return DIM == 3 && sizeof(S) == 2; }
+constexpr bool tgv_pairs() { return DIM == 3 && (sizeof(S) == 2 || std::is_same<S, float>::value); }
 
 // Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
 // the point) are passed in when the caller carries them from the previous
