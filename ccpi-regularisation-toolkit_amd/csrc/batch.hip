@@ -810,9 +810,8 @@ public:
         j->nz = j->dim == 2 ? 1 : pj.dimZ;
         j->n = (size_t)pj.dimX * pj.dimY * j->nz;
         j->wide = opts_.wide_index || !fits_u32(j->n, sizeof(float));
-        // 3D TGV state in pair fields spans 2 n values of S per field
-        if (j->pub.method == CILREG_TGV && j->dim == 3 && tgv_pairs<3, S>() && !fits_u32(j->n, 2 * sizeof(S)))
-            j->wide = true;
+        // TGV state in pair fields spans 2 n values of S per field
+        if (j->pub.method == CILREG_TGV && tgv_pairs<3, S>() && !fits_u32(j->n, 2 * sizeof(S))) j->wide = true;
         j->ty = j->dim == 3 ? 8 : 4;
         // two-iteration passes: FGP, PD and TGV in 3D, FGP on small 2D images
         const bool two = (j->pub.method != CILREG_ROF && j->dim == 3) ||
@@ -1134,12 +1133,16 @@ private:
                 break;
             default:                                                                       // U x2, (V, P, Q) x2
                 res(nT); res(nT);
-                if (tgv_pairs<3, S>() && d == 3) {
-                    // pair fields (tgv_pairs): channels 2k and 2k + 1 of a set share one buffer of 2 n values
-                    for (int k = 0; k < 2 * 6; ++k) {
-                        const size_t o = ar.reserve(2 * nS);
-                        j.off_fld[m++] = o;
-                        j.off_fld[m++] = o;
+                if (tgv_pairs<3, S>()) {
+                    // pair fields (tgv_pairs): channels 2k and 2k + 1 of a set share one buffer of
+                    // 2 n values; in 2D the seventh channel (Q12) has a field of its own
+                    for (int b = 0; b < 2; ++b) {
+                        for (int k = 0; k + 1 < 2 * d + nq; k += 2) {
+                            const size_t o = ar.reserve(2 * nS);
+                            j.off_fld[m++] = o;
+                            j.off_fld[m++] = o;
+                        }
+                        if (d == 2) res(nS);
                     }
                 } else {
                     for (int i = 0; i < 2 * (2 * d + nq); ++i) res(nS);
@@ -1216,8 +1219,9 @@ private:
                 break;
             default:
                 launch_copy(j.fld[0], j.f, nT, s_);
-                if (tgv_pairs<3, S>() && d == 3) {   // set 0 of V, P, Q: six pair fields
-                    for (int i = 0; i < 2 * d + nq; i += 2) launch_zero(j.fld[2 + i], 2 * nS, s_);
+                if (tgv_pairs<3, S>()) {   // set 0 of V, P, Q: pair fields (and Q12 alone in 2D)
+                    for (int i = 0; i + 1 < 2 * d + nq; i += 2) launch_zero(j.fld[2 + i], 2 * nS, s_);
+                    if (d == 2) launch_zero(j.fld[2 + 6], nS, s_);
                 } else {
                     for (int i = 0; i < 2 * d + nq; ++i) launch_zero(j.fld[2 + i], nS, s_);
                 }

@@ -81,8 +81,8 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
         return 0;
     }
     const int nq = dim == 3 ? 6 : 3;
-    // 3D state in FP32 or 16-bit storage lives in pair fields (two channels side by side, tgv_pairs)
-    const bool pairs = tgv_pairs<3, S>() && dim == 3;
+    // state in FP32 or 16-bit storage lives in pair fields (two channels side by side, tgv_pairs)
+    const bool pairs = tgv_pairs<3, S>();
     const size_t pes = pairs ? 2 * sizeof(S) : sizeof(S);   // bytes per point of one field
     const bool small = !opts.wide_index && fits_u32(n, sizeof(T) > pes ? sizeof(T) : pes);
     const int ty = (opts.tile_y == 4 || dim == 2) ? 4 : 8;
@@ -105,12 +105,19 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     size_t oU[2], oV[2][3], oP[2][3], oQ[2][6], oCtrl, oPart;
     for (int b = 0; b < 2; ++b) {
         oU[b] = ar.reserve(n * sizeof(T));
-        if (pairs) {   // six pair fields; the second channel of a pair names the same buffer
+        if (pairs && dim == 3) {   // six pair fields; the second channel of a pair names the same buffer
             const size_t pb = 2 * n * sizeof(S);
             oV[b][0] = oV[b][1] = ar.reserve(pb);    // V1, V2
             oV[b][2] = oP[b][0] = ar.reserve(pb);    // V3, P1
             oP[b][1] = oP[b][2] = ar.reserve(pb);    // P2, P3
             for (int c = 0; c < 6; c += 2) oQ[b][c] = oQ[b][c + 1] = ar.reserve(pb);
+        } else if (pairs) {   // 2D: three pair fields and Q12 alone
+            const size_t pb = 2 * n * sizeof(S);
+            oV[b][0] = oV[b][1] = oV[b][2] = ar.reserve(pb);   // V1, V2
+            oP[b][0] = oP[b][1] = oP[b][2] = ar.reserve(pb);   // P1, P2
+            oQ[b][0] = oQ[b][1] = ar.reserve(pb);              // Q11, Q22
+            oQ[b][2] = ar.reserve(n * sizeof(S));              // Q12
+            for (int c = 3; c < 6; ++c) oQ[b][c] = oQ[b][2];
         } else {
             for (int c = 0; c < dim; ++c) oV[b][c] = ar.reserve(n * sizeof(S));
             for (int c = 0; c < dim; ++c) oP[b][c] = ar.reserve(n * sizeof(S));
@@ -123,10 +130,16 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     // State channel ch (V1..3, P1..3, then Q) of set b: its first element and
     // element stride.
     auto chan = [&](int b, int ch, int& stride) -> S* {
-        if (pairs) {
+        if (pairs && dim == 3) {
             const size_t pf[6] = {oV[b][0], oV[b][2], oP[b][1], oQ[b][0], oQ[b][2], oQ[b][4]};
             stride = 2;
             return (S*)ar.at(pf[ch / 2]) + (ch & 1);
+        }
+        if (pairs) {   // 2D channels: V 0-1, P 3-4, Q 6-8
+            stride = ch == 8 ? 1 : 2;
+            if (ch < 3) return (S*)ar.at(oV[b][0]) + ch;
+            if (ch < 6) return (S*)ar.at(oP[b][0]) + (ch - 3);
+            return ch == 8 ? (S*)ar.at(oQ[b][2]) : (S*)ar.at(oQ[b][0]) + (ch - 6);
         }
         stride = 1;
         return (S*)ar.at(ch < 3 ? oV[b][ch] : ch < 6 ? oP[b][ch - 3] : oQ[b][ch - 6]);
@@ -134,11 +147,16 @@ int tgv_solve(const T* d_f, T* d_out, T lambda, T alpha1, T alpha0, int iter, T 
     const int nch = 2 * dim + nq;   // channels in state order (3D: 12, 2D: 7)
     auto chan_of = [&](int k) { return k < dim ? k : k < 2 * dim ? 3 + (k - dim) : 6 + (k - 2 * dim); };
     launch_copy(ar.at(oU[0]), d_f, n * sizeof(T), s);
-    if (pairs) {
+    if (pairs && dim == 3) {
         for (int k = 0; k < 6; ++k) {
             int st;
             launch_zero(chan(0, 2 * k, st), 2 * n * sizeof(S), s);
         }
+    } else if (pairs) {
+        launch_zero(ar.at(oV[0][0]), 2 * n * sizeof(S), s);
+        launch_zero(ar.at(oP[0][0]), 2 * n * sizeof(S), s);
+        launch_zero(ar.at(oQ[0][0]), 2 * n * sizeof(S), s);
+        launch_zero(ar.at(oQ[0][2]), n * sizeof(S), s);
     } else {
         for (int c = 0; c < dim; ++c) {
             launch_zero(ar.at(oV[0][c]), n * sizeof(S), s);

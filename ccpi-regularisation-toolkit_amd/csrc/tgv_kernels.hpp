@@ -93,15 +93,17 @@ __device__ inline void tgv_dual_math(T sigma, T alpha1, T alpha0, bool lx, bool 
     }
 }
 
-// 3D state in FP32 or 16-bit storage keeps two components of a point side
-// by side (pair fields, the first component at the even element): (V1, V2)
+// State in FP32 or 16-bit storage keeps two components of a point side by
+// side (pair fields, the first component at the even element).  3D: (V1, V2)
 // in V[0], (V3, P1) in V[2], (P2, P3) in P[1], (Q11, Q22) in Q[0], (Q33, Q12)
-// in Q[2], (Q13, Q23) in Q[4]; the other component pointers are unused.  A
-// point then issues 11 loads of a pair instead of 18 single loads and 6
-// stores instead of 12 (192x512x512: FP32 0.87 -> 0.79 ms, BF16 0.71 -> 0.62
-// ms per iteration).  FP64 state keeps one field per component.
+// in Q[2], (Q13, Q23) in Q[4]; 2D: (V1, V2) in V[0], (P1, P2) in P[0],
+// (Q11, Q22) in Q[0] and Q12 alone in Q[2].  The other component pointers
+// are unused.  A 3D point then issues 11 loads of a pair instead of 18
+// single loads and 6 stores instead of 12 (192x512x512: FP32 0.87 -> 0.79
+// ms, BF16 0.71 -> 0.62 ms per iteration).  FP64 state keeps one field per
+// component.
 template <int DIM, typename S>
-constexpr bool tgv_pairs() { return DIM == 3 && (sizeof(S) == 2 || std::is_same<S, float>::value); }
+constexpr bool tgv_pairs() { return (DIM == 2 || DIM == 3) && (sizeof(S) == 2 || std::is_same<S, float>::value); }
 
 // Updated and projected duals at (px, py, z).  u and v (u_bar and v_bar at
 // the point) are passed in when the caller carries them from the previous
@@ -117,7 +119,23 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
     const I i = (I)z * sxy + (I)py * nx + (I)px;
     const bool lx = px == g.nx - 1, ly = py == g.ny - 1, lz = (DIM == 2) || z == g.nz - 1;
     const I ix = lx ? i : i + 1, iy = ly ? i : i + nx, iz = lz ? i : i + sxy;
-    if constexpr (tgv_pairs<DIM, S>()) {
+    if constexpr (tgv_pairs<DIM, S>() && DIM == 2) {
+        if (!carried) {
+            u = ldx<T>(a.U, i);
+            ldpair<T>(a.V[0], i, v[0], v[1]);
+        }
+        const T ux = ldx<T>(a.U, ix), uy = ldx<T>(a.U, iy);
+        uz = (T)0;
+        T vx[3] = {0, 0, 0}, vy[3] = {0, 0, 0}, po[3] = {0, 0, 0}, qo[6] = {0, 0, 0, 0, 0, 0};
+        ldpair<T>(a.V[0], ix, vx[0], vx[1]);
+        ldpair<T>(a.V[0], iy, vy[0], vy[1]);
+        ldpair<T>(a.P[0], i, po[0], po[1]);
+        ldpair<T>(a.Q[0], i, qo[0], qo[1]);
+        qo[2] = ldx<T>(a.Q[2], i);
+        tgv_dual_math<DIM, T>(a.sigma, a.alpha1, a.alpha0, lx, ly, lz, u, v, ux, uy, uz, vx, vy, vz, po, qo, p, q);
+        return;
+    }
+    if constexpr (tgv_pairs<DIM, S>() && DIM == 3) {
         T v3i, p1i, unused;
         ldpair<T>(a.V[2], i, v3i, p1i);
         if (!carried) {
@@ -167,7 +185,12 @@ __device__ inline void tgv_dual_at(const TgvArgs<T, S>& a, int px, int py, int z
 template <int DIM, typename I, typename T, typename S, class A>
 __device__ inline void tgv_store_state(const A& a, I i, T ubn, const T vbn[3], const T p[3], const T q[6]) {
     stnt<T>(a.Uout, i, ubn);
-    if constexpr (tgv_pairs<DIM, S>()) {
+    if constexpr (tgv_pairs<DIM, S>() && DIM == 2) {
+        stpair_nt<T>(a.Vout[0], i, vbn[0], vbn[1]);
+        stpair_nt<T>(a.Pout[0], i, p[0], p[1]);
+        stpair_nt<T>(a.Qout[0], i, q[0], q[1]);
+        stnt<T>(a.Qout[2], i, q[2]);
+    } else if constexpr (tgv_pairs<DIM, S>()) {
         stpair_nt<T>(a.Vout[0], i, vbn[0], vbn[1]);
         stpair_nt<T>(a.Vout[2], i, vbn[2], p[0]);
         stpair_nt<T>(a.Pout[1], i, p[1], p[2]);
